@@ -1,0 +1,159 @@
+/*
+ * ldga_hip.h -- C ABI of libldga_hip.so, the MI355X (gfx950) ladder hot path
+ * of LadderDGA.jl v0.15.1: bubble, Bethe-Salpeter solve, EoM self-energy and
+ * the lambda-correction sums.
+ *
+ * Conventions (all entry points):
+ *   - Arrays are column-major with Julia's index order, exactly as the
+ *     reference stores them; "complex" arrays are interleaved (re, im)
+ *     float64 pairs (Julia ComplexF64).
+ *   - k arrays run over the irreducible wedge given to ldga_set_kgrid; the G
+ *     tables run over the full Ns^D grid (first axis fastest), then over
+ *     fft_range (origin freq->fft_first).
+ *   - Functions without the _dev suffix take HOST pointers (Julia arrays):
+ *     they upload, compute, download and synchronise.  The _dev variants take
+ *     DEVICE pointers and a hipStream_t (passed as void*, NULL = the
+ *     context's own stream); they are asynchronous.
+ *   - Return value: LDGA_OK (0) or a negative LDGA_ERR_* code; the message is
+ *     available from ldga_last_error().  The library never frees caller
+ *     memory.  A context must be used from one host thread at a time.
+ */
+#ifndef LDGA_HIP_H
+#define LDGA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LDGA_OK 0
+#define LDGA_ERR_SHAPE (-1)      /* sizes inconsistent or above a kernel limit   */
+#define LDGA_ERR_HIP (-2)        /* a HIP runtime call failed                    */
+#define LDGA_ERR_ZERO_PIVOT (-3) /* BSE matrix exactly singular; outputs hold NaN */
+#define LDGA_ERR_ARG (-4)        /* invalid argument (NULL pointer, bad flag)    */
+#define LDGA_ERR_STATE (-5)      /* call order (e.g. no k grid set)              */
+
+/* BSE precision modes */
+#define LDGA_PREC_FP64 0  /* FP64 Gauss-Jordan elimination                     */
+#define LDGA_PREC_MIXED 1 /* FP32 factorisation + FP64 iterative refinement    */
+
+/* EoM kernel selectors: src/ladderDGATools.jl:260-266 (eom, eom_chi_m, ...) */
+#define LDGA_EOM_FULL 0
+#define LDGA_EOM_CHI_M 1
+#define LDGA_EOM_GAMMA_M 2
+#define LDGA_EOM_CHI_D 3
+#define LDGA_EOM_GAMMA_D 4
+#define LDGA_EOM_REST_01 5
+#define LDGA_EOM_REST 6
+
+typedef struct ldga_ctx ldga_ctx;
+
+/* SimulationParameters (src/Config.jl:66-75) as far as the path needs it. */
+typedef struct {
+  int32_t n_iw;        /* sP.n_iω                                        */
+  int32_t n_inu;       /* sP.n_iν                                        */
+  int32_t n_inu_shell; /* sP.n_iν_shell                                  */
+  int32_t shift;       /* sP.shift (0/1)                                 */
+  int32_t fft_first;   /* first(sP.fft_range)                            */
+  int32_t n_fft;       /* length(sP.fft_range)                           */
+} ldga_freq;
+
+/* ModelParameters (src/Config.jl:39-47) fields used by the path. */
+typedef struct {
+  double U, mu, beta, n, Ekin_DMFT;
+} ldga_model;
+
+/* Per-call BSE report. */
+typedef struct {
+  int64_t n_matrices;    /* matrices factorised                              */
+  int64_t n_zero_pivot;  /* matrices with an exactly zero pivot              */
+  int32_t ir_steps_max;  /* mixed mode: largest refinement step count used   */
+  int32_t n_fallback;    /* mixed mode: matrices re-solved in FP64           */
+  double max_rel_resid;  /* mixed mode: largest final |r|/|b| (FP64)         */
+} ldga_bse_report;
+
+/* ---- context --------------------------------------------------------- */
+int ldga_version(void);
+/* New context on HIP device `device` (own non-blocking stream, default priority). */
+int ldga_create(int device, ldga_ctx **out);
+int ldga_destroy(ldga_ctx *ctx);
+const char *ldga_last_error(const ldga_ctx *ctx);
+/* Synchronise the context's stream (host-array calls already do). */
+int ldga_sync(ldga_ctx *ctx);
+
+/* ---- k grid -----------------------------------------------------------
+ * Replaces the KGrid of Dispersions.jl used at src/BSETools.jl:165,
+ * src/ladderDGATools.jl:151-157 and DataTypes.jl:240-246 (kintegrate).
+ * D in {2,3}; Ns even, Ns <= 64 (2D) / 32 (3D).  wedge[d + D*q] are the
+ * integer grid coordinates (0..Ns/2, descending over d) of wedge point q;
+ * kMult is derived from them.  eps[q] is the dispersion on the wedge. */
+int ldga_set_kgrid(ldga_ctx *ctx, int D, int Ns, int Nq, const int32_t *wedge, const double *eps);
+/* The library's own wedge for (D, Ns): fills wedge (D*Nq ints) and kmult (Nq)
+ * when non-NULL and returns Nq (> 0) or an error code. */
+int ldga_default_wedge(int D, int Ns, int32_t *wedge, double *kmult);
+
+/* ---- bubble: calc_bubble (src/BSETools.jl:157-172) ---------------------
+ * chi0[q, nu, w] (Nq x 2(n_inu+shell) x (2 n_iw+1)) = -beta conv_k(G(nu), G(nu+w)).
+ * G_fft = fft(G_k), G_rfft = fft(reverse(G_k)) (src/helpers.jl:164-183). */
+int ldga_bubble(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double *G_fft, const double *G_rfft,
+                double *chi0);
+int ldga_bubble_dev(ldga_ctx *ctx, const ldga_freq *fr, double beta, const void *G_fft, const void *G_rfft,
+                    void *chi0, void *stream);
+
+/* ---- Bethe-Salpeter: calc_chi_gamma (src/BSETools.jl:192-248) -----------
+ * Replaces the per-(q,w) LapackWrapper.inv! (src/LapackWrapper.jl:64-72,
+ * zgetrf_+zgetri_) and the plain sums: chi = Re sum(A^-1)/beta^2,
+ * gamma[nu'] = sum_nu A^-1[nu,nu'] / (chi0[q,nu',w](1 + s U chi)), with
+ * A = Gamma[:,:,w] + diag(1/chi0[q, shell+1:shell+Nnu, w]).
+ * s = +1 for :m, -1 for :d.  Gamma is (Nnu x Nnu x Nw), chi (Nq x Nw) float64,
+ * gamma (Nq x Nnu x Nw).  report may be NULL. */
+int ldga_chi_gamma(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double U, double beta,
+                   const double *Gamma, const double *chi0, double *chi, double *gamma, ldga_bse_report *report);
+int ldga_chi_gamma_dev(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double U, double beta,
+                       const void *Gamma, const void *chi0, void *chi, void *gamma, ldga_bse_report *report,
+                       void *stream);
+
+/* ---- lambda_0: calc_λ0 plain branch (src/BSETools.jl:128-141) ----------
+ * lambda0[q,nu',w] = chi0[q, shell+nu', w]/beta^2 * sum_nu F[nu,nu',w]. */
+int ldga_lambda0(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double *F, const double *chi0,
+                 double *lambda0);
+int ldga_lambda0_dev(ldga_ctx *ctx, const ldga_freq *fr, double beta, const void *F, const void *chi0,
+                     void *lambda0, void *stream);
+
+/* ---- bosonic sums: sum_kω (src/DataTypes.jl:215-249) with chi_lambda ----
+ * (LambdaCorrection/helpers.jl:18,61).  For chi (Nq x Nw) over the usable
+ * range w_lo..w_hi (0-based, inclusive):
+ *   out[0] = sum_kω(chi_λ(chi, lam))
+ *   out[1] = sum_kω(dchi_λ(chi, lam)) exactly as the reference's df_c1
+ *            (transform = dchi_λ, tail terms included; conditions_new.jl:153)
+ *   out[2] = d out[0] / d lam  (the true derivative, no tail terms)
+ * with tail coefficient c2 (chi.tail_c[3]); lam = 0 means no transform.
+ * out_dev of the _dev form is a device array of 3 doubles. */
+int ldga_sum_kw(ldga_ctx *ctx, const double *chi, int Nw, int w_lo, int w_hi, double c2, double beta, double lam,
+                double out[3]);
+int ldga_sum_kw_dev(ldga_ctx *ctx, const void *chi, int Nw, int w_lo, int w_hi, double c2, double beta,
+                    double lam, void *out_dev, void *stream);
+
+/* ---- EoM self-energy: calc_Σ (src/ladderDGATools_singleCore.jl:114-136) -
+ * Sigma[k, nu] (Nq x numax, nu = 0..numax-1) =
+ *   (1/beta) sum_w conv_q(K(q,nu,w), G(k+q, nu+w)) + tail/(i nu_n) + n U/2,
+ * K = eom(U, gamma_m, gamma_d, chi_m^λm, chi_d^λd, lambda0)
+ * (src/ladderDGATools.jl:260), tail = -U (sum_kω(chi_m^λm) - chiloc_m_sum)
+ * when tc != 0.  eom_kind selects one of the split kernels (LDGA_EOM_*) used
+ * by calc_Σ_parts; for those the Hartree term is added only to REST and the
+ * tail only when tc != 0 and eom_kind == FULL.  chi_m/chi_d (Nq x Nw) are
+ * read-only: the lambda map is applied on the fly and never stored. */
+int ldga_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m, double lam_d,
+               int tc, int eom_kind, double chiloc_m_sum, const double *chi_m, const double *gamma_m,
+               const double *chi_d, const double *gamma_d, const double *lambda0, const double *G_rfft,
+               double *sigma);
+int ldga_sigma_dev(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m, double lam_d,
+                   int tc, int eom_kind, double chiloc_m_sum, const void *chi_m, const void *gamma_m,
+                   const void *chi_d, const void *gamma_d, const void *lambda0, const void *G_rfft, void *sigma,
+                   void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LDGA_HIP_H */

@@ -1,0 +1,264 @@
+// Batched Bethe-Salpeter solve (calc_χγ, src/BSETools.jl:192-248).
+//
+// The reference inverts A = Gamma_r(w) + diag(1/chi0(q,.,w)) for every (q,w)
+// with zgetrf+zgetri (src/LapackWrapper.jl:64-72) and then only uses the
+// column sums of A^-1:
+//     chi       = Re sum_{nu nu'} A^-1 / beta^2
+//     gamma[nu'] = sum_nu A^-1[nu,nu'] / (chi0[q,nu',w] (1 + s U chi))
+// i.e. the row vector y^T = 1^T A^-1, the solution of A^T y = 1.  Each
+// workgroup solves one (q,w) system by Gauss-Jordan elimination with partial
+// pivoting on [A^T | 1] held entirely in registers:
+//   * thread (tr, tc) owns rows tr + 16a and columns tc + TC b of the padded
+//     system (the right-hand side is column N);
+//   * column k is owned by 16 lanes of one wave: the pivot search is a
+//     16-lane butterfly of (|re|+|im|, row) -- LAPACK's izamax criterion --
+//     and those lanes publish the multipliers through LDS (one barrier per
+//     column);
+//   * the pivot row is exchanged wave-locally through LDS without a
+//     workgroup barrier, because all owners of a column block sit in the
+//     same wave;
+//   * rows are never swapped: pivoted rows are masked out of later pivot
+//     searches and every other row is eliminated, so after N steps row p_k
+//     holds piv_k * y_k = b_{p_k}.
+// Partial pivoting picks the same pivot sequence as zgetrf (up to ties).
+#include <algorithm>
+#include <cstring>
+
+#include "ldga_internal.h"
+
+namespace {
+
+// Element b (wave-uniform, not a compile-time constant) of a register row, as
+// a chain of selects.  The empty asm keeps the compiler from folding the chain
+// back into a dynamically indexed array, which would move the whole register
+// tile to scratch memory.
+template <int CPT>
+__device__ __forceinline__ double2 sel(const double (&re)[CPT], const double (&im)[CPT], int b) {
+  double vr = re[0], vi = im[0];
+#pragma unroll
+  for (int i = 1; i < CPT; ++i) {
+    const bool m = (i == b);
+    vr = m ? re[i] : vr;
+    vi = m ? im[i] : vi;
+    __asm__ volatile("" : "+v"(vr), "+v"(vi));
+  }
+  return cz(vr, vi);
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wave's LDS stores have landed
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int RPT, int CPT, int TC>
+__global__ __launch_bounds__(16 * TC) void bse_gj_f64_kernel(int N, int Nq, int Nc, int shell,
+                                                              const double2 *__restrict__ Gam,
+                                                              const double2 *__restrict__ chi0, double s, double U,
+                                                              double beta, double *__restrict__ chi_out,
+                                                              double2 *__restrict__ gamma_out,
+                                                              unsigned long long *__restrict__ nzero) {
+  constexpr int NT = 16 * TC, NW = NT / 64, ROWS = 16 * RPT;
+  __shared__ double2 lbuf[2][ROWS];
+  __shared__ double2 prow[NW][4][CPT];
+  __shared__ int piv_row[ROWS];
+  __shared__ double2 piv_val[ROWS];
+  __shared__ double2 bvec[ROWS];
+  __shared__ double red[NW];
+
+  const int t = threadIdx.x, tr = t & 15, tc = t >> 4, wave = t >> 6, tcl = (t >> 4) & 3;
+  const int q = blockIdx.x % Nq, w = blockIdx.x / Nq;
+  const double2 *G = Gam + (size_t)w * N * N;
+
+  // B = A^T: B[i][j] = A[j][i] = Gamma[j, i, w] (+ 1/chi0 on the diagonal); column N = 1
+  // real and imaginary parts in separate plain arrays so that they stay in VGPRs
+  double Br[RPT][CPT], Bi[RPT][CPT];
+  unsigned pmask = 0;
+#pragma unroll
+  for (int a = 0; a < RPT; ++a) {
+    const int i = tr + 16 * a;
+    if (i >= N) pmask |= 1u << a;
+#pragma unroll
+    for (int b = 0; b < CPT; ++b) {
+      const int j = tc + TC * b;
+      double2 v = cz(0.0, 0.0);
+      if (i < N) {
+        if (j < N)
+          v = G[j + (size_t)N * i];
+        else if (j == N)
+          v = cz(1.0, 0.0);
+        if (i == j) v = cadd(v, crecip(chi0[q + (size_t)Nq * (shell + i + (size_t)Nc * w)]));
+      }
+      Br[a][b] = v.x;
+      Bi[a][b] = v.y;
+    }
+  }
+
+  for (int k = 0; k < N; ++k) {
+    const int kc = k % TC, bk = k / TC;
+    if (tc == kc) {
+      // ---- pivot search over the unpivoted rows of column k (16 lanes) ----
+      double bc = -1.0;
+      int bi = 0x7fffffff;
+      double2 bv = cz(0.0, 0.0);
+#pragma unroll
+      for (int a = 0; a < RPT; ++a) {
+        const double2 v = sel<CPT>(Br[a], Bi[a], bk);
+        double c = fabs(v.x) + fabs(v.y);
+        if (!(c >= 0.0)) c = 0.0;  // NaN input: still pick a row, NaN propagates
+        if (!((pmask >> a) & 1u) && c > bc) {
+          bc = c;
+          bi = tr + 16 * a;
+          bv = v;
+        }
+      }
+#pragma unroll
+      for (int off = 8; off > 0; off >>= 1) {
+        const double oc = __shfl_xor(bc, off);
+        const int oi = __shfl_xor(bi, off);
+        const double ox = __shfl_xor(bv.x, off), oy = __shfl_xor(bv.y, off);
+        if (oc > bc || (oc == bc && oi < bi)) {
+          bc = oc;
+          bi = oi;
+          bv = cz(ox, oy);
+        }
+      }
+      const double2 inv = crecip(bv);
+#pragma unroll
+      for (int a = 0; a < RPT; ++a) {
+        const int i = tr + 16 * a;
+        const double2 v = sel<CPT>(Br[a], Bi[a], bk);
+        lbuf[k & 1][i] = (i == bi) ? cz(0.0, 0.0) : cmul(v, inv);
+      }
+      if (tr == 0) {
+        piv_row[k] = bi;
+        piv_val[k] = bv;
+        if (bc == 0.0) atomicAdd(nzero, 1ull);
+      }
+    }
+    __syncthreads();
+    // ---- elimination of column k from every other row -------------------
+    const int p = piv_row[k];
+    const int ap = p >> 4, trp = p & 15;
+    if (tr == trp) {
+      pmask |= 1u << ap;
+#pragma unroll
+      for (int b = 0; b < CPT; ++b) {
+        double vr = Br[0][b], vi = Bi[0][b];
+#pragma unroll
+        for (int a = 1; a < RPT; ++a) {
+          const bool m = (a == ap);
+          vr = m ? Br[a][b] : vr;
+          vi = m ? Bi[a][b] : vi;
+          __asm__ volatile("" : "+v"(vr), "+v"(vi));
+        }
+        prow[wave][tcl][b] = cz(vr, vi);
+      }
+    }
+    wave_lds_sync();
+    double2 u[CPT];
+#pragma unroll
+    for (int b = 0; b < CPT; ++b) u[b] = prow[wave][tcl][b];
+#pragma unroll
+    for (int a = 0; a < RPT; ++a) {
+      const double2 l = lbuf[k & 1][tr + 16 * a];
+#pragma unroll
+      for (int b = 0; b < CPT; ++b) {
+        if (tc + TC * b > k) {
+          double xr = Br[a][b], xi = Bi[a][b];
+          xr = fma(-l.x, u[b].x, xr);
+          xr = fma(l.y, u[b].y, xr);
+          xi = fma(-l.x, u[b].y, xi);
+          xi = fma(-l.y, u[b].x, xi);
+          Br[a][b] = xr;
+          Bi[a][b] = xi;
+        }
+      }
+    }
+  }
+
+  // ---- y_k = b_{p_k} / piv_k, chi and gamma -------------------------------
+  if (tc == N % TC) {
+#pragma unroll
+    for (int a = 0; a < RPT; ++a) bvec[tr + 16 * a] = sel<CPT>(Br[a], Bi[a], N / TC);
+  }
+  __syncthreads();
+  double part = 0.0;
+  for (int k = t; k < N; k += NT) {
+    const double2 y = cdiv(bvec[piv_row[k]], piv_val[k]);
+    lbuf[0][k] = y;
+    part += y.x;
+  }
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+  if ((t & 63) == 0) red[wave] = part;
+  __syncthreads();
+  double tot = 0.0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) tot += red[i];
+  const double chi = tot / (beta * beta);
+  const double fac = 1.0 + s * U * chi;
+  for (int k = t; k < N; k += NT) {
+    const double2 c0 = chi0[q + (size_t)Nq * (k + (size_t)Nc * w)];  // no shell offset (BSETools.jl:236)
+    gamma_out[q + (size_t)Nq * (k + (size_t)N * w)] = cdiv(lbuf[0][k], cscale(c0, fac));
+  }
+  if (t == 0) chi_out[q + (size_t)Nq * w] = chi;
+}
+
+template <int RPT, int CPT, int TC>
+int run_gj(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam, const double2 *chi0, double s,
+           double U, double beta, double *chi, double2 *gamma, unsigned long long *nzero, hipStream_t st) {
+  if (N > 16 * RPT || N + 1 > TC * CPT) return ldga_fail(ctx, LDGA_ERR_SHAPE, "BSE tile too small for N=%d", N);
+  bse_gj_f64_kernel<RPT, CPT, TC><<<dim3((unsigned)Nq * Nw), dim3(16 * TC), 0, st>>>(N, Nq, Nc, shell, Gam, chi0, s, U,
+                                                                                   beta, chi, gamma, nzero);
+  LDGA_HIP_TRY(ctx, hipGetLastError());
+  return LDGA_OK;
+}
+
+}  // namespace
+
+int launch_bse_mixed(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam, const double2 *chi0,
+                     double s, double U, double beta, double *chi, double2 *gamma, unsigned long long *counters,
+                     hipStream_t st);
+
+int launch_bse(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double U, double beta, const double2 *Gam,
+               const double2 *chi0, double *chi, double2 *gamma, ldga_bse_report *rep, hipStream_t st) {
+  if (s != 1 && s != -1) return ldga_fail(ctx, LDGA_ERR_ARG, "channel sign s must be +1 (m) or -1 (d)");
+  const int N = 2 * fr->n_inu, Nw = 2 * fr->n_iw + 1, Nq = ctx->Nq, Nc = N + 2 * fr->n_inu_shell;
+  // counters: [0] zero pivots, [1] refinement steps max, [2] FP64 fallbacks, [3] max residual (bits)
+  unsigned long long *cnt = (unsigned long long *)ldga_buf(ctx, "bse_counters", 4 * 8);
+  if (!cnt) return LDGA_ERR_HIP;
+  LDGA_HIP_TRY(ctx, hipMemsetAsync(cnt, 0, 4 * 8, st));
+  int rc;
+  if (precision == LDGA_PREC_FP64) {
+    if (N <= 47)
+      rc = run_gj<3, 3, 16>(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, st);
+    else if (N <= 79)
+      rc = run_gj<5, 3, 32>(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, st);
+    else if (N <= 95)
+      rc = run_gj<6, 3, 32>(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, st);
+    else if (N <= 127)
+      rc = run_gj<8, 4, 32>(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, st);
+    else
+      return ldga_fail(ctx, LDGA_ERR_SHAPE, "FP64 BSE supports Nnu <= 127 (got %d)", N);
+  } else if (precision == LDGA_PREC_MIXED) {
+    rc = launch_bse_mixed(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, st);
+  } else {
+    return ldga_fail(ctx, LDGA_ERR_ARG, "unknown precision mode %d", precision);
+  }
+  if (rc) return rc;
+  if (rep) {
+    unsigned long long h[4];
+    LDGA_HIP_TRY(ctx, hipMemcpyAsync(h, cnt, sizeof h, hipMemcpyDeviceToHost, st));
+    LDGA_HIP_TRY(ctx, hipStreamSynchronize(st));
+    rep->n_matrices = (int64_t)Nq * Nw;
+    rep->n_zero_pivot = (int64_t)h[0];
+    rep->ir_steps_max = (int32_t)h[1];
+    rep->n_fallback = (int32_t)h[2];
+    double r;
+    memcpy(&r, &h[3], 8);
+    rep->max_rel_resid = r;
+    if (h[0]) return ldga_fail(ctx, LDGA_ERR_ZERO_PIVOT, "%llu BSE matrices had an exactly zero pivot", h[0]);
+  }
+  return LDGA_OK;
+}

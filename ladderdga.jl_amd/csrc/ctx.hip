@@ -1,0 +1,423 @@
+// Context, k-grid tables, buffer cache and the host-array (Julia-facing) entry
+// points of libldga_hip.so.  The host-array forms stage through cached device
+// buffers and call the same launchers as the _dev forms.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+
+#include "ldga_internal.h"
+
+int ldga_fail(ldga_ctx *ctx, int code, const char *fmt, ...) {
+  if (ctx) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    ctx->err = buf;
+  }
+  return code;
+}
+
+void *ldga_buf(ldga_ctx *ctx, const char *role, size_t bytes) {
+  DevBuf &b = ctx->bufs[role];
+  if (b.bytes >= bytes && b.ptr) return b.ptr;
+  if (b.ptr) {
+    hipStreamSynchronize(ctx->stream);
+    hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  if (hipMalloc(&b.ptr, std::max<size_t>(bytes, 256)) != hipSuccess) {
+    b.ptr = nullptr;
+    ldga_fail(ctx, LDGA_ERR_HIP, "hipMalloc of %zu bytes for %s failed", bytes, role);
+    return nullptr;
+  }
+  b.bytes = bytes;
+  return b.ptr;
+}
+
+hipStream_t ldga_stream(ldga_ctx *ctx, void *stream) { return stream ? (hipStream_t)stream : ctx->stream; }
+
+int ldga_check_freq(ldga_ctx *ctx, const ldga_freq *fr) {
+  if (!fr) return ldga_fail(ctx, LDGA_ERR_ARG, "freq is NULL");
+  if (fr->n_iw < 0 || fr->n_inu < 1 || fr->n_inu_shell < 0 || fr->n_fft < 1 || (fr->shift != 0 && fr->shift != 1))
+    return ldga_fail(ctx, LDGA_ERR_SHAPE, "invalid frequency parameters");
+  return LDGA_OK;
+}
+
+extern "C" {
+
+int ldga_version(void) { return 10000; }
+
+int ldga_create(int device, ldga_ctx **out) {
+  if (!out) return LDGA_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return LDGA_ERR_HIP;
+  ldga_ctx *ctx = new ldga_ctx();
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return LDGA_ERR_HIP;
+  }
+  *out = ctx;
+  return LDGA_OK;
+}
+
+static void free_kgrid(ldga_ctx *ctx) {
+  const void *p[] = {ctx->oct.oct2wedge, ctx->oct.wedge2oct, ctx->oct.octfull, ctx->oct.octw,
+                     ctx->oct.octphase, ctx->oct.cosT,      ctx->oct.kmult};
+  for (const void *q : p)
+    if (q) hipFree(const_cast<void *>(q));
+  ctx->oct = OctTables{};
+  ctx->has_kgrid = false;
+}
+
+int ldga_destroy(ldga_ctx *ctx) {
+  if (!ctx) return LDGA_OK;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  free_kgrid(ctx);
+  for (auto &kv : ctx->bufs)
+    if (kv.second.ptr) hipFree(kv.second.ptr);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return LDGA_OK;
+}
+
+const char *ldga_last_error(const ldga_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int ldga_sync(ldga_ctx *ctx) {
+  if (!ctx) return LDGA_ERR_ARG;
+  LDGA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return LDGA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// k grid
+// ---------------------------------------------------------------------------
+static int canon_key(const int *c, int D, int n) {
+  int s[3] = {0, 0, 0};
+  for (int d = 0; d < D; ++d) s[d] = c[d];
+  std::sort(s, s + D, [](int a, int b) { return a > b; });
+  int key = 0;
+  for (int d = D - 1; d >= 0; --d) key = key * n + s[d];
+  return key;
+}
+
+int ldga_default_wedge(int D, int Ns, int32_t *wedge, double *kmult) {
+  if ((D != 2 && D != 3) || Ns < 2 || (Ns & 1)) return LDGA_ERR_SHAPE;
+  const int n = Ns / 2 + 1;
+  int Nq = 0;
+  // descending coordinates c0 >= c1 (>= c2), enumerated with c0 slowest
+  std::vector<int> pts;
+  if (D == 2) {
+    for (int a = 0; a < n; ++a)
+      for (int b = 0; b <= a; ++b) pts.insert(pts.end(), {a, b});
+  } else {
+    for (int a = 0; a < n; ++a)
+      for (int b = 0; b <= a; ++b)
+        for (int c = 0; c <= b; ++c) pts.insert(pts.end(), {a, b, c});
+  }
+  Nq = (int)pts.size() / D;
+  if (wedge)
+    for (int i = 0; i < Nq * D; ++i) wedge[i] = pts[i];
+  if (kmult) {
+    std::map<int, int> idx;
+    for (int q = 0; q < Nq; ++q) idx[canon_key(&pts[q * D], D, n)] = q;
+    std::fill(kmult, kmult + Nq, 0.0);
+    int c[3] = {0, 0, 0};
+    const int Nk = D == 2 ? Ns * Ns : Ns * Ns * Ns;
+    for (int f = 0; f < Nk; ++f) {
+      int r = f;
+      for (int d = 0; d < D; ++d) {
+        int x = r % Ns;
+        r /= Ns;
+        c[d] = std::min(x, Ns - x);
+      }
+      kmult[idx[canon_key(c, D, n)]] += 1.0;
+    }
+  }
+  return Nq;
+}
+
+int ldga_set_kgrid(ldga_ctx *ctx, int D, int Ns, int Nq, const int32_t *wedge, const double *eps) {
+  if (!ctx || !wedge) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  if ((D != 2 && D != 3) || Ns < 2 || (Ns & 1) || (D == 2 && Ns > 64) || (D == 3 && Ns > 32))
+    return ldga_fail(ctx, LDGA_ERR_SHAPE, "k grid D=%d Ns=%d outside the supported range (2D Ns<=64, 3D Ns<=32)", D, Ns);
+  const int n = Ns / 2 + 1;
+  const int NO = D == 2 ? n * n : n * n * n;
+  const int Nexp = ldga_default_wedge(D, Ns, nullptr, nullptr);
+  if (Nq != Nexp) return ldga_fail(ctx, LDGA_ERR_SHAPE, "wedge has %d points, expected %d", Nq, Nexp);
+  hipSetDevice(ctx->device);
+  // wedge lookup by canonical key
+  std::map<int, int> idx;
+  for (int q = 0; q < Nq; ++q) {
+    for (int d = 0; d < D; ++d) {
+      int c = wedge[d + D * q];
+      if (c < 0 || c >= n) return ldga_fail(ctx, LDGA_ERR_SHAPE, "wedge coordinate %d of point %d outside [0,Ns/2]", c, q);
+      if (d > 0 && c > wedge[d - 1 + D * q])
+        return ldga_fail(ctx, LDGA_ERR_SHAPE, "wedge point %d coordinates are not descending", q);
+    }
+    int key = canon_key(&wedge[D * q], D, n);
+    if (idx.count(key)) return ldga_fail(ctx, LDGA_ERR_SHAPE, "wedge point %d repeated", q);
+    idx[key] = q;
+  }
+  std::vector<int32_t> o2w(NO), w2o(Nq), ofull(NO);
+  std::vector<double> ow(NO), cosT(Ns), kmult(Nq, 0.0);
+  std::vector<double2> oph(NO);
+  const int Nk = D == 2 ? Ns * Ns : Ns * Ns * Ns;
+  for (int o = 0; o < NO; ++o) {
+    int c[3] = {o % n, (o / n) % n, D == 3 ? o / (n * n) : 0};
+    o2w[o] = idx.at(canon_key(c, D, n));
+    ofull[o] = c[0] + Ns * c[1] + (D == 3 ? Ns * Ns * c[2] : 0);
+    double w = 1.0;
+    int sum = 0;
+    for (int d = 0; d < D; ++d) {
+      w *= (c[d] == 0 || c[d] == Ns / 2) ? 1.0 : 2.0;
+      sum += c[d];
+    }
+    ow[o] = w;
+    const double ang = -2.0 * M_PI * (double)(sum % Ns) / (double)Ns;
+    oph[o] = make_double2(std::cos(ang), std::sin(ang));
+  }
+  for (int q = 0; q < Nq; ++q) {
+    const int *c = &wedge[D * q];
+    w2o[q] = c[0] + n * c[1] + (D == 3 ? n * n * c[2] : 0);
+  }
+  for (int m = 0; m < Ns; ++m) cosT[m] = std::cos(2.0 * M_PI * (double)m / (double)Ns);
+  {
+    int c[3];
+    for (int f = 0; f < Nk; ++f) {
+      int r = f;
+      for (int d = 0; d < D; ++d) {
+        int x = r % Ns;
+        r /= Ns;
+        c[d] = std::min(x, Ns - x);
+      }
+      kmult[idx.at(canon_key(c, D, n))] += 1.0;
+    }
+  }
+  free_kgrid(ctx);
+  OctTables &T = ctx->oct;
+  auto up = [&](const void *src, size_t bytes, void **dst) -> int {
+    LDGA_HIP_TRY(ctx, hipMalloc(dst, bytes));
+    LDGA_HIP_TRY(ctx, hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return LDGA_OK;
+  };
+  int rc = 0;
+  void *p;
+  if ((rc = up(o2w.data(), NO * 4, &p))) return rc;
+  T.oct2wedge = (int32_t *)p;
+  if ((rc = up(w2o.data(), Nq * 4, &p))) return rc;
+  T.wedge2oct = (int32_t *)p;
+  if ((rc = up(ofull.data(), NO * 4, &p))) return rc;
+  T.octfull = (int32_t *)p;
+  if ((rc = up(ow.data(), NO * 8, &p))) return rc;
+  T.octw = (double *)p;
+  if ((rc = up(oph.data(), NO * 16, &p))) return rc;
+  T.octphase = (double2 *)p;
+  if ((rc = up(cosT.data(), Ns * 8, &p))) return rc;
+  T.cosT = (double *)p;
+  if ((rc = up(kmult.data(), Nq * 8, &p))) return rc;
+  T.kmult = (double *)p;
+  T.D = D;
+  T.Ns = Ns;
+  T.n = n;
+  T.NO = NO;
+  T.Nq = Nq;
+  T.Nk = (double)Nk;
+  T.stride[0] = 1;
+  T.stride[1] = n;
+  T.stride[2] = n * n;
+  ctx->D = D;
+  ctx->Ns = Ns;
+  ctx->Nq = Nq;
+  ctx->n = n;
+  ctx->NO = NO;
+  ctx->h_wedge.assign(wedge, wedge + D * Nq);
+  ctx->h_kmult = kmult;
+  ctx->h_eps.assign(Nq, 0.0);
+  if (eps) ctx->h_eps.assign(eps, eps + Nq);
+  ctx->has_kgrid = true;
+  return LDGA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// host-array entry points
+// ---------------------------------------------------------------------------
+#define NEED_KGRID(ctx) \
+  if (!(ctx)->has_kgrid) return ldga_fail((ctx), LDGA_ERR_STATE, "ldga_set_kgrid has not been called")
+
+static int h2d(ldga_ctx *ctx, const char *role, const void *src, size_t bytes, void **dst) {
+  *dst = ldga_buf(ctx, role, bytes);
+  if (!*dst) return LDGA_ERR_HIP;
+  LDGA_HIP_TRY(ctx, hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  return LDGA_OK;
+}
+
+static int d2h(ldga_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  LDGA_HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  LDGA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return LDGA_OK;
+}
+
+int ldga_bubble(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double *G_fft, const double *G_rfft,
+                double *chi0) {
+  if (!ctx || !G_fft || !G_rfft || !chi0) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  NEED_KGRID(ctx);
+  int rc = ldga_check_freq(ctx, fr);
+  if (rc) return rc;
+  hipSetDevice(ctx->device);
+  const size_t gbytes = (size_t)ctx->oct.Nk * fr->n_fft * 16;
+  const size_t obytes = (size_t)ctx->Nq * 2 * (fr->n_inu + fr->n_inu_shell) * (2 * fr->n_iw + 1) * 16;
+  void *a, *b, *o;
+  if ((rc = h2d(ctx, "G_fft", G_fft, gbytes, &a))) return rc;
+  if ((rc = h2d(ctx, "G_rfft", G_rfft, gbytes, &b))) return rc;
+  if (!(o = ldga_buf(ctx, "chi0", obytes))) return LDGA_ERR_HIP;
+  if ((rc = launch_bubble(ctx, fr, beta, (const double2 *)a, (const double2 *)b, (double2 *)o, ctx->stream))) return rc;
+  return d2h(ctx, chi0, o, obytes);
+}
+
+int ldga_bubble_dev(ldga_ctx *ctx, const ldga_freq *fr, double beta, const void *G_fft, const void *G_rfft,
+                    void *chi0, void *stream) {
+  if (!ctx || !G_fft || !G_rfft || !chi0) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  NEED_KGRID(ctx);
+  int rc = ldga_check_freq(ctx, fr);
+  if (rc) return rc;
+  hipSetDevice(ctx->device);
+  return launch_bubble(ctx, fr, beta, (const double2 *)G_fft, (const double2 *)G_rfft, (double2 *)chi0,
+                       ldga_stream(ctx, stream));
+}
+
+int ldga_chi_gamma(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double U, double beta,
+                   const double *Gamma, const double *chi0, double *chi, double *gamma, ldga_bse_report *report) {
+  if (!ctx || !Gamma || !chi0 || !chi || !gamma) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  NEED_KGRID(ctx);
+  int rc = ldga_check_freq(ctx, fr);
+  if (rc) return rc;
+  hipSetDevice(ctx->device);
+  const size_t Nnu = 2 * (size_t)fr->n_inu, Nw = 2 * (size_t)fr->n_iw + 1, Nq = ctx->Nq;
+  const size_t gb = Nnu * Nnu * Nw * 16, cb = Nq * (Nnu + 2 * fr->n_inu_shell) * Nw * 16;
+  void *g, *c, *xo, *yo;
+  if ((rc = h2d(ctx, "Gamma", Gamma, gb, &g))) return rc;
+  if ((rc = h2d(ctx, "chi0", chi0, cb, &c))) return rc;
+  if (!(xo = ldga_buf(ctx, "chi", Nq * Nw * 8))) return LDGA_ERR_HIP;
+  if (!(yo = ldga_buf(ctx, "gamma", Nq * Nnu * Nw * 16))) return LDGA_ERR_HIP;
+  int st = launch_bse(ctx, s, precision, fr, U, beta, (const double2 *)g, (const double2 *)c, (double *)xo,
+                      (double2 *)yo, report, ctx->stream);
+  if (st && st != LDGA_ERR_ZERO_PIVOT) return st;
+  if ((rc = d2h(ctx, chi, xo, Nq * Nw * 8))) return rc;
+  if ((rc = d2h(ctx, gamma, yo, Nq * Nnu * Nw * 16))) return rc;
+  return st;
+}
+
+int ldga_chi_gamma_dev(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double U, double beta,
+                       const void *Gamma, const void *chi0, void *chi, void *gamma, ldga_bse_report *report,
+                       void *stream) {
+  if (!ctx || !Gamma || !chi0 || !chi || !gamma) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  NEED_KGRID(ctx);
+  int rc = ldga_check_freq(ctx, fr);
+  if (rc) return rc;
+  hipSetDevice(ctx->device);
+  return launch_bse(ctx, s, precision, fr, U, beta, (const double2 *)Gamma, (const double2 *)chi0, (double *)chi,
+                    (double2 *)gamma, report, ldga_stream(ctx, stream));
+}
+
+int ldga_lambda0(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double *F, const double *chi0,
+                 double *lambda0) {
+  if (!ctx || !F || !chi0 || !lambda0) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  NEED_KGRID(ctx);
+  int rc = ldga_check_freq(ctx, fr);
+  if (rc) return rc;
+  hipSetDevice(ctx->device);
+  const size_t Nnu = 2 * (size_t)fr->n_inu, Nw = 2 * (size_t)fr->n_iw + 1, Nq = ctx->Nq;
+  const size_t fb = Nnu * Nnu * Nw * 16, cb = Nq * (Nnu + 2 * fr->n_inu_shell) * Nw * 16, ob = Nq * Nnu * Nw * 16;
+  void *f, *c, *o;
+  if ((rc = h2d(ctx, "F", F, fb, &f))) return rc;
+  if ((rc = h2d(ctx, "chi0", chi0, cb, &c))) return rc;
+  if (!(o = ldga_buf(ctx, "lambda0", ob))) return LDGA_ERR_HIP;
+  if ((rc = launch_lambda0(ctx, fr, beta, (const double2 *)f, (const double2 *)c, (double2 *)o, ctx->stream)))
+    return rc;
+  return d2h(ctx, lambda0, o, ob);
+}
+
+int ldga_lambda0_dev(ldga_ctx *ctx, const ldga_freq *fr, double beta, const void *F, const void *chi0,
+                     void *lambda0, void *stream) {
+  if (!ctx || !F || !chi0 || !lambda0) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  NEED_KGRID(ctx);
+  int rc = ldga_check_freq(ctx, fr);
+  if (rc) return rc;
+  hipSetDevice(ctx->device);
+  return launch_lambda0(ctx, fr, beta, (const double2 *)F, (const double2 *)chi0, (double2 *)lambda0,
+                        ldga_stream(ctx, stream));
+}
+
+int ldga_sum_kw(ldga_ctx *ctx, const double *chi, int Nw, int w_lo, int w_hi, double c2, double beta, double lam,
+                double out[3]) {
+  if (!ctx || !chi || !out) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  NEED_KGRID(ctx);
+  hipSetDevice(ctx->device);
+  void *c, *o;
+  int rc;
+  if ((rc = h2d(ctx, "sum_chi", chi, (size_t)ctx->Nq * Nw * 8, &c))) return rc;
+  if (!(o = ldga_buf(ctx, "sum_out", 3 * 8))) return LDGA_ERR_HIP;
+  if ((rc = launch_sum_kw(ctx, (const double *)c, Nw, w_lo, w_hi, c2, beta, lam, (double *)o, ctx->stream)))
+    return rc;
+  return d2h(ctx, out, o, 3 * 8);
+}
+
+int ldga_sum_kw_dev(ldga_ctx *ctx, const void *chi, int Nw, int w_lo, int w_hi, double c2, double beta,
+                    double lam, void *out_dev, void *stream) {
+  if (!ctx || !chi || !out_dev) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  NEED_KGRID(ctx);
+  hipSetDevice(ctx->device);
+  return launch_sum_kw(ctx, (const double *)chi, Nw, w_lo, w_hi, c2, beta, lam, (double *)out_dev,
+                       ldga_stream(ctx, stream));
+}
+
+int ldga_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m, double lam_d,
+               int tc, int eom_kind, double chiloc_m_sum, const double *chi_m, const double *gamma_m,
+               const double *chi_d, const double *gamma_d, const double *lambda0, const double *G_rfft,
+               double *sigma) {
+  if (!ctx || !mp || !chi_m || !gamma_m || !chi_d || !gamma_d || !lambda0 || !G_rfft || !sigma)
+    return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  NEED_KGRID(ctx);
+  int rc = ldga_check_freq(ctx, fr);
+  if (rc) return rc;
+  hipSetDevice(ctx->device);
+  const size_t Nnu = 2 * (size_t)fr->n_inu, Nw = 2 * (size_t)fr->n_iw + 1, Nq = ctx->Nq;
+  const size_t xb = Nq * Nw * 8, yb = Nq * Nnu * Nw * 16, gb = (size_t)ctx->oct.Nk * fr->n_fft * 16;
+  void *cm, *gm, *cd, *gd, *l0, *gr, *so;
+  if ((rc = h2d(ctx, "chi_m", chi_m, xb, &cm))) return rc;
+  if ((rc = h2d(ctx, "chi_d", chi_d, xb, &cd))) return rc;
+  if ((rc = h2d(ctx, "gamma_m", gamma_m, yb, &gm))) return rc;
+  if ((rc = h2d(ctx, "gamma_d", gamma_d, yb, &gd))) return rc;
+  if ((rc = h2d(ctx, "lambda0", lambda0, yb, &l0))) return rc;
+  if ((rc = h2d(ctx, "G_rfft", G_rfft, gb, &gr))) return rc;
+  if (!(so = ldga_buf(ctx, "sigma", Nq * (size_t)std::max(numax, 1) * 16))) return LDGA_ERR_HIP;
+  if ((rc = launch_sigma(ctx, fr, mp, numax, lam_m, lam_d, tc, eom_kind, chiloc_m_sum, (const double *)cm,
+                         (const double2 *)gm, (const double *)cd, (const double2 *)gd, (const double2 *)l0,
+                         (const double2 *)gr, (double2 *)so, ctx->stream)))
+    return rc;
+  return d2h(ctx, sigma, so, Nq * (size_t)numax * 16);
+}
+
+int ldga_sigma_dev(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m, double lam_d,
+                   int tc, int eom_kind, double chiloc_m_sum, const void *chi_m, const void *gamma_m,
+                   const void *chi_d, const void *gamma_d, const void *lambda0, const void *G_rfft, void *sigma,
+                   void *stream) {
+  if (!ctx || !mp || !chi_m || !gamma_m || !chi_d || !gamma_d || !lambda0 || !G_rfft || !sigma)
+    return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  NEED_KGRID(ctx);
+  int rc = ldga_check_freq(ctx, fr);
+  if (rc) return rc;
+  hipSetDevice(ctx->device);
+  return launch_sigma(ctx, fr, mp, numax, lam_m, lam_d, tc, eom_kind, chiloc_m_sum, (const double *)chi_m,
+                      (const double2 *)gamma_m, (const double *)chi_d, (const double2 *)gamma_d,
+                      (const double2 *)lambda0, (const double2 *)G_rfft, (double2 *)sigma, ldga_stream(ctx, stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,401 @@
+// k-space convolutions of the ladder path on the real-space octant:
+//   * bubble  chi0(q,nu,w)  = -beta conv_k(G(nu), G(nu+w))      (BSETools.jl:157-172)
+//   * EoM     Sigma(k,nu)   = 1/beta sum_w conv_q(K, G(nu+w))   (ladderDGATools_singleCore.jl:49-81)
+//   * lambda0 (plain)                                           (BSETools.jl:128-141)
+//   * sum_kw with the chi_lambda map                            (DataTypes.jl:215-249)
+//
+// conv(a,b)[q] = (1/Nk) sum_k a(k) b(k+q).  With A = fft(a) and B = fft(b)
+// (B(-R) read from the reversed-table fft(reverse(b)) times exp(-2 pi i sum R/Ns))
+// conv[q] = 1/Nk^2 sum_R A(R) B(-R) exp(-i q.R).  All functions of the path
+// are even in every axis, so the sum folds onto the octant R in [0,Ns/2]^D
+// with multiplicity w(R) and a separable cosine kernel: one small dense
+// "mode product" per axis, evaluated in LDS.  The last axis is evaluated only
+// at the wedge points that are stored.
+#include <algorithm>
+
+#include "ldga_internal.h"
+
+namespace {
+
+constexpr int kNT = 256;  // threads per octant workgroup
+
+// In-place mode product along axis `ax` over the whole octant held in LDS:
+//   buf[o] <- sum_r cos(2 pi q_ax r / Ns) * buf[o with coordinate ax = r]
+template <int MAXPT>
+__device__ __forceinline__ void oct_pass(double2 *buf, const double *cs, int ax, const OctTables &T) {
+  const int stride = T.stride[ax];
+  const int n = T.n, Ns = T.Ns;
+  double2 acc[MAXPT];
+#pragma unroll
+  for (int p = 0; p < MAXPT; ++p) {
+    const int o = threadIdx.x + p * kNT;
+    double2 s = cz(0.0, 0.0);
+    if (o < T.NO) {
+      const int qd = (o / stride) % n;
+      const int base = o - qd * stride;
+      int m = 0;
+      for (int r = 0; r < n; ++r) {
+        const double c = cs[m];
+        const double2 v = buf[base + r * stride];
+        s.x = fma(c, v.x, s.x);
+        s.y = fma(c, v.y, s.y);
+        m += qd;
+        m -= (m >= Ns) ? Ns : 0;
+      }
+    }
+    acc[p] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < MAXPT; ++p) {
+    const int o = threadIdx.x + p * kNT;
+    if (o < T.NO) buf[o] = acc[p];
+  }
+  __syncthreads();
+}
+
+// The last axis at one wedge point: sum_r cos(2 pi q_last r/Ns) * buf[...]
+__device__ __forceinline__ double2 oct_last_at(const double2 *buf, const double *cs, int q, const OctTables &T) {
+  const int ax = T.D - 1;
+  const int stride = T.stride[ax];
+  const int o = T.wedge2oct[q];
+  const int qd = (o / stride) % T.n;
+  const int base = o - qd * stride;
+  double2 s = cz(0.0, 0.0);
+  int m = 0;
+  for (int r = 0; r < T.n; ++r) {
+    const double c = cs[m];
+    const double2 v = buf[base + r * stride];
+    s.x = fma(c, v.x, s.x);
+    s.y = fma(c, v.y, s.y);
+    m += qd;
+    m -= (m >= T.Ns) ? T.Ns : 0;
+  }
+  return s;
+}
+
+__device__ __forceinline__ int trunc_half(int x) { return x / 2; }  // C division truncates toward 0 like Julia trunc
+
+// ---------------------------------------------------------------------------
+// bubble: one workgroup per (nu, w) pair
+// ---------------------------------------------------------------------------
+template <int MAXPT>
+__global__ __launch_bounds__(kNT) void bubble_kernel(OctTables T, int Nnu, int n_inu_tot, int n_iw, int shift,
+                                                     int fft_first, const double2 *__restrict__ Gfft,
+                                                     const double2 *__restrict__ Grfft, double2 *__restrict__ chi0,
+                                                     double scale) {
+  extern __shared__ double2 smem[];
+  double2 *buf = smem;
+  double *cs = reinterpret_cast<double *>(smem + T.NO);
+  const int pair = blockIdx.x;
+  const int nui = pair % Nnu, wi = pair / Nnu;
+  const int wn = wi - n_iw;
+  const int nun = -n_inu_tot + nui - trunc_half(shift * wn);
+  const size_t fs = (size_t)T.Nk;
+  const double2 *a = Gfft + (size_t)(nun - fft_first) * fs;
+  const double2 *b = Grfft + (size_t)(nun + wn - fft_first) * fs;
+  for (int m = threadIdx.x; m < T.Ns; m += kNT) cs[m] = T.cosT[m];
+  for (int o = threadIdx.x; o < T.NO; o += kNT) {
+    const int f = T.octfull[o];
+    const double2 p = cmul(cmul(a[f], b[f]), T.octphase[o]);
+    buf[o] = cscale(p, T.octw[o]);
+  }
+  __syncthreads();
+  for (int ax = 0; ax < T.D - 1; ++ax) oct_pass<MAXPT>(buf, cs, ax, T);
+  double2 *out = chi0 + (size_t)T.Nq * (nui + (size_t)Nnu * wi);
+  for (int q = threadIdx.x; q < T.Nq; q += kNT) out[q] = cscale(oct_last_at(buf, cs, q, T), scale);
+}
+
+// ---------------------------------------------------------------------------
+// EoM: partial sums over an omega chunk, then the inverse transform per nu
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 eom_value(int kind, double U, double2 gm, double2 gd, double cm, double cd,
+                                             double2 l0) {
+  // src/ladderDGATools.jl:260-266
+  switch (kind) {
+    case LDGA_EOM_CHI_M:
+      return cscale(gm, U * 1.5 * U * cm);
+    case LDGA_EOM_GAMMA_M:
+      return cscale(gm, U * 1.5);
+    case LDGA_EOM_CHI_D:
+      return cscale(gd, -U * 0.5 * (-U * cd));
+    case LDGA_EOM_GAMMA_D:
+      return cscale(gd, -U * 0.5);
+    case LDGA_EOM_REST_01:
+      return cz(-U, 0.0);
+    case LDGA_EOM_REST:
+      return cscale(l0, U);
+    default: {
+      const double am = 1.5 * (1.0 + U * cm), ad = 0.5 * (1.0 - U * cd);
+      return cscale(cz(gm.x * am - gd.x * ad - 1.5 + 0.5 + l0.x, gm.y * am - gd.y * ad + l0.y), U);
+    }
+  }
+}
+
+template <int MAXPT>
+__global__ __launch_bounds__(kNT) void eom_accum_kernel(OctTables T, int Nw, int Nnu, int n_inu, int n_iw, int shift,
+                                                        int fft_first, int numax, int wchunk, int kind, double U,
+                                                        double lam_m, double lam_d, const double *__restrict__ chim,
+                                                        const double *__restrict__ chid,
+                                                        const double2 *__restrict__ gm,
+                                                        const double2 *__restrict__ gd,
+                                                        const double2 *__restrict__ l0,
+                                                        const double2 *__restrict__ Grfft,
+                                                        double2 *__restrict__ partial) {
+  extern __shared__ double2 smem[];
+  double2 *buf = smem;
+  double *cs = reinterpret_cast<double *>(smem + T.NO);
+  const int nuii = blockIdx.x % numax;
+  const int chunk = blockIdx.x / numax;
+  for (int m = threadIdx.x; m < T.Ns; m += kNT) cs[m] = T.cosT[m];
+  double2 S[MAXPT];
+#pragma unroll
+  for (int p = 0; p < MAXPT; ++p) S[p] = cz(0.0, 0.0);
+  const int w0 = chunk * wchunk, w1 = min(Nw, w0 + wchunk);
+  const size_t fs = (size_t)T.Nk;
+  for (int wi = w0; wi < w1; ++wi) {
+    const int wn = wi - n_iw;
+    // nu-window of gamma at this w: nu0 index (1-based) = n_inu + shift*trunc(wn/2) + 1  (helpers.jl:58)
+    const int nui = n_inu + shift * trunc_half(wn) + nuii;  // 0-based index of nu_n = nuii
+    if (nui > 2 * n_inu - 1) continue;                     // nu list ends at 2 n_inu (singleCore.jl:59-60)
+    __syncthreads();                                        // previous iteration finished with buf
+    for (int o = threadIdx.x; o < T.NO; o += kNT) {
+      const int q = T.oct2wedge[o];
+      double cm = chim[q + (size_t)T.Nq * wi], cd = chid[q + (size_t)T.Nq * wi];
+      if (lam_m != 0.0) cm = cm / (lam_m * cm + 1.0);
+      if (lam_d != 0.0) cd = cd / (lam_d * cd + 1.0);
+      const size_t idx = q + (size_t)T.Nq * (nui + (size_t)Nnu * wi);
+      const double2 K = eom_value(kind, U, gm[idx], gd[idx], cm, cd, l0[idx]);
+      buf[o] = cscale(K, T.octw[o]);
+    }
+    __syncthreads();
+    for (int ax = 0; ax < T.D; ++ax) oct_pass<MAXPT>(buf, cs, ax, T);
+    const double2 *g = Grfft + (size_t)(nuii + wn - fft_first) * fs;
+#pragma unroll
+    for (int p = 0; p < MAXPT; ++p) {
+      const int o = threadIdx.x + p * kNT;
+      if (o < T.NO) {
+        const double2 gb = cmul(g[T.octfull[o]], T.octphase[o]);
+        S[p] = cadd(S[p], cscale(cmul(buf[o], gb), T.octw[o]));
+      }
+    }
+  }
+  double2 *dst = partial + (size_t)T.NO * (nuii + (size_t)numax * chunk);
+#pragma unroll
+  for (int p = 0; p < MAXPT; ++p) {
+    const int o = threadIdx.x + p * kNT;
+    if (o < T.NO) dst[o] = S[p];
+  }
+}
+
+template <int MAXPT>
+__global__ __launch_bounds__(kNT) void eom_final_kernel(OctTables T, int numax, int nchunks,
+                                                        const double2 *__restrict__ partial, double scale, double U,
+                                                        double beta, double hartree, int add_tail,
+                                                        const double *__restrict__ tail_sum, double chiloc_m_sum,
+                                                        double2 *__restrict__ sigma) {
+  extern __shared__ double2 smem[];
+  double2 *buf = smem;
+  double *cs = reinterpret_cast<double *>(smem + T.NO);
+  const int nuii = blockIdx.x;
+  for (int m = threadIdx.x; m < T.Ns; m += kNT) cs[m] = T.cosT[m];
+  for (int o = threadIdx.x; o < T.NO; o += kNT) {
+    double2 s = cz(0.0, 0.0);
+    for (int c = 0; c < nchunks; ++c) s = cadd(s, partial[(size_t)T.NO * (nuii + (size_t)numax * c) + o]);
+    buf[o] = s;
+  }
+  __syncthreads();
+  for (int ax = 0; ax < T.D - 1; ++ax) oct_pass<MAXPT>(buf, cs, ax, T);
+  // tail/(i nu_n) = -i * tail * beta / ((2n+1) pi)   (singleCore.jl:78-79)
+  double tail = 0.0;
+  if (add_tail) tail = -U * (tail_sum[0] - chiloc_m_sum);
+  const double tail_im = -tail * beta / ((2.0 * nuii + 1.0) * M_PI);
+  double2 *out = sigma + (size_t)T.Nq * nuii;
+  for (int q = threadIdx.x; q < T.Nq; q += kNT) {
+    const double2 v = cscale(oct_last_at(buf, cs, q, T), scale);
+    out[q] = cz(v.x + hartree, v.y + tail_im);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// lambda0 (plain sums): fsum[nu',w] = sum_nu F[nu,nu',w]; lambda0 = chi0 fsum / beta^2
+// ---------------------------------------------------------------------------
+__global__ void fsum_kernel(int Nnu, int Nw, const double2 *__restrict__ F, double2 *__restrict__ fsum) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;  // (nu', w)
+  if (col >= Nnu * Nw) return;
+  const double2 *c = F + (size_t)Nnu * col;
+  double2 s = cz(0.0, 0.0);
+  for (int v = 0; v < Nnu; ++v) s = cadd(s, c[v]);
+  fsum[col] = s;
+}
+
+__global__ void lambda0_kernel(int Nq, int Nnu, int Nw, int shell, double inv_b2, const double2 *__restrict__ fsum,
+                               const double2 *__restrict__ chi0, double2 *__restrict__ l0) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)Nq * Nnu * Nw;
+  if (i >= total) return;
+  const int q = (int)(i % Nq);
+  const size_t col = i / Nq;  // nu' + Nnu w
+  const int nu = (int)(col % Nnu), w = (int)(col / Nnu);
+  const double2 c = chi0[q + (size_t)Nq * (shell + nu + (size_t)(Nnu + 2 * shell) * w)];
+  l0[i] = cscale(cmul(fsum[col], c), inv_b2);
+}
+
+// ---------------------------------------------------------------------------
+// sum_kw: one workgroup, FP64 tree reduction
+// ---------------------------------------------------------------------------
+constexpr int kSumNT = 1024;
+__global__ __launch_bounds__(kSumNT) void sum_kw_kernel(const double *__restrict__ chi, const double *__restrict__ kmult,
+                                                        int Nq, int w_lo, int w_hi, double Nk, double lam, double c2,
+                                                        double beta, int n_iw, double *__restrict__ out) {
+  __shared__ double red[2][kSumNT / 64];
+  double s0 = 0.0, s1 = 0.0;
+  const int nw = w_hi - w_lo + 1;
+  const size_t total = (size_t)Nq * nw;
+  for (size_t i = threadIdx.x; i < total; i += kSumNT) {
+    const int q = (int)(i % Nq);
+    const int w = w_lo + (int)(i / Nq);
+    const double x = chi[q + (size_t)Nq * w];
+    const double f = (lam != 0.0) ? x / (lam * x + 1.0) : x;
+    s0 = fma(kmult[q], f, s0);
+    s1 = fma(kmult[q], -f * f, s1);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off);
+    s1 += __shfl_xor(s1, off);
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wv] = s0;
+    red[1][wv] = s1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0, tail = 0.0;
+    for (int i = 0; i < kSumNT / 64; ++i) {
+      a += red[0][i];
+      b += red[1][i];
+    }
+    // Re(c2/(i w_n)^2) = -c2 beta^2 / (4 pi^2 m^2), zero at m = 0  (DataTypes.jl:223-225)
+    for (int w = w_lo; w <= w_hi; ++w) {
+      const int m = w - n_iw;
+      if (m != 0) tail += -c2 * beta * beta / (4.0 * M_PI * M_PI * (double)m * (double)m);
+    }
+    const double konst = -beta * beta / 12.0 * c2;
+    out[0] = (a / Nk - tail + konst) / beta;
+    out[1] = (b / Nk - tail + konst) / beta;
+    out[2] = (b / Nk) / beta;
+  }
+}
+
+}  // namespace
+
+// number of register slots per thread needed for the octant (NO / 256 rounded up)
+static int oct_maxpt(int NO) { return (NO + kNT - 1) / kNT; }
+
+// Launch KERNEL<MAXPT> with the smallest register slot count that covers the
+// octant; large octants (3D) need more than the default 64 KiB of LDS.
+#define OCT_DISPATCH(NO, SMEM, KERNEL, GRID, STREAM, ...)                                          \
+  do {                                                                                             \
+    const int mp_ = oct_maxpt(NO);                                                                 \
+    auto go_ = [&](auto kern) -> int {                                                             \
+      if ((SMEM) > 65536)                                                                          \
+        LDGA_HIP_TRY(ctx, hipFuncSetAttribute((const void *)kern,                                  \
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)(SMEM))); \
+      kern<<<(GRID), kNT, (SMEM), (STREAM)>>>(__VA_ARGS__);                                        \
+      LDGA_HIP_TRY(ctx, hipGetLastError());                                                        \
+      return LDGA_OK;                                                                              \
+    };                                                                                             \
+    int rc_;                                                                                       \
+    if (mp_ <= 5)                                                                                  \
+      rc_ = go_(KERNEL<5>);                                                                        \
+    else if (mp_ <= 9)                                                                             \
+      rc_ = go_(KERNEL<9>);                                                                        \
+    else if (mp_ <= 20)                                                                            \
+      rc_ = go_(KERNEL<20>);                                                                       \
+    else                                                                                           \
+      rc_ = ldga_fail(ctx, LDGA_ERR_SHAPE, "octant of %d points is too large", NO);                \
+    if (rc_) return rc_;                                                                           \
+  } while (0)
+
+int launch_bubble(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double2 *Gfft, const double2 *Grfft,
+                  double2 *chi0, hipStream_t st) {
+  const OctTables &T = ctx->oct;
+  const int ntot = fr->n_inu + fr->n_inu_shell;
+  const int Nnu = 2 * ntot, Nw = 2 * fr->n_iw + 1;
+  // every (nu, nu+w) must lie inside fft_range
+  int lo = 1 << 30, hi = -(1 << 30);
+  for (int wn = -fr->n_iw; wn <= fr->n_iw; ++wn) {
+    const int a = -ntot - (fr->shift * wn) / 2, b = ntot - 1 - (fr->shift * wn) / 2;
+    lo = std::min(lo, std::min(a, a + wn));
+    hi = std::max(hi, std::max(b, b + wn));
+  }
+  if (lo < fr->fft_first || hi > fr->fft_first + fr->n_fft - 1)
+    return ldga_fail(ctx, LDGA_ERR_SHAPE, "fft_range [%d,%d] does not cover nu, nu+w in [%d,%d]", fr->fft_first,
+                     fr->fft_first + fr->n_fft - 1, lo, hi);
+  const size_t smem = (size_t)T.NO * 16 + (size_t)T.Ns * 8;
+  const double scale = -beta / (T.Nk * T.Nk);
+  OCT_DISPATCH(T.NO, smem, bubble_kernel, Nnu * Nw, st, T, Nnu, ntot, fr->n_iw, fr->shift, fr->fft_first, Gfft,
+               Grfft, chi0, scale);
+  return LDGA_OK;
+}
+
+int launch_lambda0(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double2 *F, const double2 *chi0,
+                   double2 *lambda0, hipStream_t st) {
+  const int Nnu = 2 * fr->n_inu, Nw = 2 * fr->n_iw + 1, Nq = ctx->Nq;
+  double2 *fsum = (double2 *)ldga_buf(ctx, "fsum", (size_t)Nnu * Nw * 16);
+  if (!fsum) return LDGA_ERR_HIP;
+  fsum_kernel<<<(Nnu * Nw + 255) / 256, 256, 0, st>>>(Nnu, Nw, F, fsum);
+  const size_t total = (size_t)Nq * Nnu * Nw;
+  lambda0_kernel<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(Nq, Nnu, Nw, fr->n_inu_shell, 1.0 / (beta * beta),
+                                                                  fsum, chi0, lambda0);
+  LDGA_HIP_TRY(ctx, hipGetLastError());
+  return LDGA_OK;
+}
+
+int launch_sum_kw(ldga_ctx *ctx, const double *chi, int Nw, int w_lo, int w_hi, double c2, double beta, double lam,
+                  double *out_dev, hipStream_t st) {
+  if (Nw < 1 || (Nw & 1) == 0 || w_lo < 0 || w_hi >= Nw || w_lo > w_hi)
+    return ldga_fail(ctx, LDGA_ERR_SHAPE, "bad omega range [%d,%d] of %d", w_lo, w_hi, Nw);
+  const OctTables &T = ctx->oct;
+  sum_kw_kernel<<<1, kSumNT, 0, st>>>(chi, T.kmult, T.Nq, w_lo, w_hi, T.Nk, lam, c2, beta, (Nw - 1) / 2, out_dev);
+  LDGA_HIP_TRY(ctx, hipGetLastError());
+  return LDGA_OK;
+}
+
+int launch_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m, double lam_d,
+                 int tc, int eom_kind, double chiloc_m_sum, const double *chi_m, const double2 *gamma_m,
+                 const double *chi_d, const double2 *gamma_d, const double2 *lambda0, const double2 *Grfft,
+                 double2 *sigma, hipStream_t st) {
+  const OctTables &T = ctx->oct;
+  const int Nnu = 2 * fr->n_inu, Nw = 2 * fr->n_iw + 1;
+  if (numax < 1 || numax > 2 * fr->n_inu) return ldga_fail(ctx, LDGA_ERR_SHAPE, "numax=%d out of range", numax);
+  if (eom_kind < LDGA_EOM_FULL || eom_kind > LDGA_EOM_REST) return ldga_fail(ctx, LDGA_ERR_ARG, "bad eom kind");
+  // G(nu + w) must lie inside fft_range for every nu in 0..numax-1
+  if (-fr->n_iw < fr->fft_first || numax - 1 + fr->n_iw > fr->fft_first + fr->n_fft - 1)
+    return ldga_fail(ctx, LDGA_ERR_SHAPE, "fft_range does not cover nu+w");
+  // the gamma window must contain nu_n = 0 at every w (helpers.jl:58)
+  if (fr->n_inu - fr->shift * (fr->n_iw / 2) < 0)
+    return ldga_fail(ctx, LDGA_ERR_SHAPE, "nu window does not contain nu_n = 0 at every w");
+  // omega chunking: enough workgroups to fill the device
+  int nchunks = std::max(1, std::min(Nw, (2048 + numax - 1) / numax));
+  const int wchunk = (Nw + nchunks - 1) / nchunks;
+  nchunks = (Nw + wchunk - 1) / wchunk;
+  double2 *partial = (double2 *)ldga_buf(ctx, "eom_partial", (size_t)T.NO * numax * nchunks * 16);
+  double *tsum = (double *)ldga_buf(ctx, "eom_tail", 3 * 8);
+  if (!partial || !tsum) return LDGA_ERR_HIP;
+  const bool add_tail = tc && eom_kind == LDGA_EOM_FULL;
+  if (add_tail) {
+    int rc = launch_sum_kw(ctx, chi_m, Nw, 0, Nw - 1, mp->Ekin_DMFT, mp->beta, lam_m, tsum, st);
+    if (rc) return rc;
+  }
+  const size_t smem = (size_t)T.NO * 16 + (size_t)T.Ns * 8;
+  OCT_DISPATCH(T.NO, smem, eom_accum_kernel, numax * nchunks, st, T, Nw, Nnu, fr->n_inu, fr->n_iw, fr->shift,
+               fr->fft_first, numax, wchunk, eom_kind, mp->U, lam_m, lam_d, chi_m, chi_d, gamma_m, gamma_d, lambda0,
+               Grfft, partial);
+  const double hartree = (eom_kind == LDGA_EOM_FULL || eom_kind == LDGA_EOM_REST) ? mp->n * mp->U / 2.0 : 0.0;
+  const double scale = 1.0 / (mp->beta * T.Nk * T.Nk);
+  OCT_DISPATCH(T.NO, smem, eom_final_kernel, numax, st, T, numax, nchunks, partial, scale, mp->U, mp->beta, hartree,
+               add_tail ? 1 : 0, tsum, chiloc_m_sum, sigma);
+  return LDGA_OK;
+}

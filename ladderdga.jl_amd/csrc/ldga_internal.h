@@ -1,0 +1,92 @@
+// Internal declarations shared by the libldga_hip.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ldga_hip.h"
+
+// ---------------------------------------------------------------------------
+// complex double helpers (double2 = (re, im), the ComplexF64 memory layout)
+// ---------------------------------------------------------------------------
+__host__ __device__ inline double2 cz(double re, double im) { return make_double2(re, im); }
+__host__ __device__ inline double2 cadd(double2 a, double2 b) { return cz(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline double2 csub(double2 a, double2 b) { return cz(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline double2 cmul(double2 a, double2 b) { return cz(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__host__ __device__ inline double2 cscale(double2 a, double s) { return cz(a.x * s, a.y * s); }
+// 1/a as conj(a)/|a|^2 (a = 0 gives NaN, as a singular pivot should)
+__host__ __device__ inline double2 crecip(double2 a) {
+  double d = a.x * a.x + a.y * a.y;
+  return cz(a.x / d, -a.y / d);
+}
+__host__ __device__ inline double2 cdiv(double2 a, double2 b) { return cmul(a, crecip(b)); }
+
+// ---------------------------------------------------------------------------
+// octant tables: the k-space convolutions run on the first octant
+// R in [0, Ns/2]^D of the real-space (FFT) grid, where every quantity of the
+// path is even; see DESIGN for the derivation.
+// ---------------------------------------------------------------------------
+struct OctTables {
+  int D, Ns, n, NO, Nq;
+  const int32_t *oct2wedge;  // NO: wedge index of the (sorted) octant point
+  const int32_t *wedge2oct;  // Nq: octant linear index of wedge point q
+  const int32_t *octfull;    // NO: linear index of the octant point in the full Ns^D grid
+  const double *octw;        // NO: multiplicity weight prod_d w(R_d), w = 1 at 0 and Ns/2, else 2
+  const double2 *octphase;   // NO: exp(-2 pi i sum_d R_d / Ns), undoes the reverse() of G_rfft
+  const double *cosT;        // Ns: cos(2 pi m / Ns)
+  const double *kmult;       // Nq: wedge multiplicities
+  double Nk;                 // Ns^D
+  int stride[3];             // octant strides 1, n, n^2
+};
+
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+struct ldga_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // k grid
+  bool has_kgrid = false;
+  int D = 0, Ns = 0, Nq = 0, n = 0, NO = 0;
+  std::vector<int32_t> h_wedge;
+  std::vector<double> h_kmult, h_eps;
+  OctTables oct{};
+  // cached device buffers (grow-only, keyed by role)
+  std::map<std::string, DevBuf> bufs;
+};
+
+// error plumbing ------------------------------------------------------------
+int ldga_fail(ldga_ctx *ctx, int code, const char *fmt, ...);
+#define LDGA_HIP_TRY(ctx, expr)                                                                             \
+  do {                                                                                                      \
+    hipError_t e_ = (expr);                                                                                 \
+    if (e_ != hipSuccess) return ldga_fail((ctx), LDGA_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                           __FILE__, __LINE__);                                             \
+  } while (0)
+
+// cached device scratch (returns nullptr on failure and sets ctx->err)
+void *ldga_buf(ldga_ctx *ctx, const char *role, size_t bytes);
+hipStream_t ldga_stream(ldga_ctx *ctx, void *stream);
+int ldga_check_freq(ldga_ctx *ctx, const ldga_freq *fr);
+
+// kernel launchers (device pointers, asynchronous) ---------------------------
+int launch_bubble(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double2 *Gfft, const double2 *Grfft,
+                  double2 *chi0, hipStream_t st);
+int launch_bse(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double U, double beta, const double2 *Gam,
+               const double2 *chi0, double *chi, double2 *gamma, ldga_bse_report *rep, hipStream_t st);
+int launch_lambda0(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double2 *F, const double2 *chi0,
+                   double2 *lambda0, hipStream_t st);
+int launch_sum_kw(ldga_ctx *ctx, const double *chi, int Nw, int w_lo, int w_hi, double c2, double beta, double lam,
+                  double *out_dev, hipStream_t st);
+int launch_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m, double lam_d,
+                 int tc, int eom_kind, double chiloc_m_sum, const double *chi_m, const double2 *gamma_m,
+                 const double *chi_d, const double2 *gamma_d, const double2 *lambda0, const double2 *Grfft,
+                 double2 *sigma, hipStream_t st);

@@ -1,0 +1,209 @@
+"""Host mirror of the reference functions on the ladder hot path.
+
+Same names, argument meaning and error behaviour as LadderDGA.jl; every
+array computation runs in libldga_hip.so on the GPU (host-array entry
+points).  Julia's in-place ``χ_λ!`` is spelled ``χ_λ_inplace``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import warnings
+
+import numpy as np
+
+from . import _lib
+from ._lib import Freq, Model, BseReport, default_context, ptr
+from .kgrid import KGrid
+from .types import ModelParameters, SimulationParameters, χ0T, χT, γT
+
+c128, f64 = np.complex128, np.float64
+
+
+def _freq(sP: SimulationParameters) -> Freq:
+    return Freq(sP.n_iω, sP.n_iν, sP.n_iν_shell, int(sP.shift), sP.fft_range.start, len(sP.fft_range))
+
+
+def _model(mP: ModelParameters) -> Model:
+    return Model(mP.U, mP.μ, mP.β, mP.n, mP.Ekin_DMFT)
+
+
+def _ctx(ctx, kG):
+    ctx = ctx or default_context()
+    ctx.set_kgrid(kG)
+    return ctx
+
+
+# ---------------------------------------------------------------------------
+# bubble (src/BSETools.jl:157-172)
+# ---------------------------------------------------------------------------
+def calc_bubble(Gνω, Gνω_r, kG: KGrid, mP: ModelParameters, sP: SimulationParameters, local_tail=False, ctx=None):
+    ctx = _ctx(ctx, kG)
+    Nν, Nω = 2 * (sP.n_iν + sP.n_iν_shell), 2 * sP.n_iω + 1
+    if Gνω.shape != kG.gridshape() + (len(sP.fft_range),) or Gνω_r.shape != Gνω.shape:
+        raise ValueError("G tables must have shape gridshape(kG) x length(fft_range)")
+    out = np.empty((kG.Nq, Nν, Nω), dtype=c128, order="F")
+    ctx.check(_lib.lib().ldga_bubble(ctx.handle, C.byref(_freq(sP)), mP.β, ptr(Gνω, c128, "Gνω"),
+                                     ptr(Gνω_r, c128, "Gνω_r"), out.ctypes.data))
+    return χ0T(out, mP.β, ν_shell_size=sP.n_iν_shell)
+
+
+# ---------------------------------------------------------------------------
+# Bethe-Salpeter (src/BSETools.jl:192-248)
+# ---------------------------------------------------------------------------
+def calc_χγ(type_: str, Γr, χ0: χ0T, kG: KGrid, mP: ModelParameters, sP: SimulationParameters,
+            precision: str = "fp64", ctx=None, report: dict | None = None):
+    if type_ == "d" or type_ == ":d":
+        s = -1
+    elif type_ == "m" or type_ == ":m":
+        s = 1
+    else:
+        raise ValueError("Unkown type")
+    ctx = _ctx(ctx, kG)
+    Nν, Nω = 2 * sP.n_iν, 2 * sP.n_iω + 1
+    if Γr.shape != (Nν, Nν, Nω):
+        raise ValueError(f"Γr must be {Nν}x{Nν}x{Nω}")
+    χ = np.empty((kG.Nq, Nω), dtype=f64, order="F")
+    γ = np.empty((kG.Nq, Nν, Nω), dtype=c128, order="F")
+    rep = BseReport()
+    prec = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}[precision]
+    rc = _lib.lib().ldga_chi_gamma(ctx.handle, s, prec, C.byref(_freq(sP)), mP.U, mP.β, ptr(Γr, c128, "Γr"),
+                                   ptr(χ0.data, c128, "χ0.data"), χ.ctypes.data, γ.ctypes.data, C.byref(rep))
+    ctx.check(rc, allow=(_lib.LDGA_ERR_ZERO_PIVOT,))
+    if rc == _lib.LDGA_ERR_ZERO_PIVOT:
+        # the reference ignores LAPACK's info (src/LapackWrapper.jl:31); keep going with NaN
+        warnings.warn(f"{rep.n_zero_pivot} BSE matrices are exactly singular; their χ, γ are NaN")
+    if report is not None:
+        report.update(n_matrices=rep.n_matrices, n_zero_pivot=rep.n_zero_pivot, ir_steps_max=rep.ir_steps_max,
+                      n_fallback=rep.n_fallback, max_rel_resid=rep.max_rel_resid)
+    return χT(χ, mP.β, tail_c=[0.0, 0.0, mP.Ekin_DMFT]), γT(γ)
+
+
+# ---------------------------------------------------------------------------
+# λ0, plain sums (src/BSETools.jl:118-144)
+# ---------------------------------------------------------------------------
+def calc_λ0(χ0: χ0T, Fr, mP: ModelParameters, sP: SimulationParameters, kG: KGrid, ctx=None):
+    ctx = _ctx(ctx, kG)
+    warnings.warn("Using plain summation for λ0, check Σ_ladder tails!")
+    Nν, Nω = 2 * sP.n_iν, 2 * sP.n_iω + 1
+    out = np.empty((kG.Nq, Nν, Nω), dtype=c128, order="F")
+    ctx.check(_lib.lib().ldga_lambda0(ctx.handle, C.byref(_freq(sP)), mP.β, ptr(Fr, c128, "Fr"),
+                                      ptr(χ0.data, c128, "χ0.data"), out.ctypes.data))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# χ_λ and the bosonic sums (LambdaCorrection/helpers.jl:18-70, DataTypes.jl:215-249)
+# ---------------------------------------------------------------------------
+def χ_λ(χ, λ: float):
+    """Elementwise χ/(λχ+1) on scalars/arrays, or a new λ-corrected χT."""
+    if isinstance(χ, χT):
+        out = χT(np.asfortranarray(χ.data / (λ * χ.data + 1)) if λ != 0 else χ.data.copy(order="F"), χ.β,
+                 tail_c=list(χ.tail_c), usable_ω=χ.usable_ω)
+        out.λ = χ.λ + λ
+        return out
+    return χ / (λ * χ + 1)
+
+
+def dχ_λ(χ, λ: float):
+    return -χ_λ(χ, λ) ** 2
+
+
+def χ_λ_inplace(χ: χT, λ: float):
+    """χ_λ!(χ, λ): maps χ.data in place and records λ (undone by reset)."""
+    if λ == 0.0:
+        return
+    if not math.isfinite(λ):
+        print(f"WARNING. SKIPPING λ correction because {λ} is not finite!")
+        return
+    χ.data[...] = χ.data / (λ * χ.data + 1)
+    χ.λ = χ.λ + λ
+
+
+def reset(χ: χT):
+    """reset!(χ): undo χ_λ! (LambdaCorrection/helpers.jl:64-70)."""
+    if χ.λ != 0:
+        lam = -χ.λ
+        χ.data[...] = χ.data / (lam * χ.data + 1)
+        χ.λ = 0.0
+
+
+def _usable(χ: χT):
+    u = list(χ.usable_ω)
+    lo, hi = u[0], u[-1]
+    if u != list(range(lo, hi + 1)):
+        raise ValueError("usable_ω must be a contiguous range")
+    return lo, hi
+
+
+def sum_kω(kG: KGrid, χ: χT, λ: float = math.nan, force_full_range=False, ctx=None, with_derivative=False):
+    """Σ_q Σ_ω χ(q, ω) with the 1/ω² tail (src/DataTypes.jl:215-249)."""
+    if χ.λ != 0 and not math.isnan(λ):
+        raise RuntimeError("χ already λ-corrected, but external λ provided!")
+    if len(χ.tail_c) == 3 and (χ.tail_c[0] != 0 or χ.tail_c[1] != 0):
+        raise RuntimeError("sum_kω only implemented for ω^2 tail!")
+    ctx = _ctx(ctx, kG)
+    lo, hi = (0, χ.data.shape[1] - 1) if force_full_range else _usable(χ)
+    if (lo, hi) != (0, χ.data.shape[1] - 1):
+        # the reference builds the 1/ω² tail over all ω but sums a usable_ω slice of χ:
+        # it only runs for the full range (src/DataTypes.jl:220-226)
+        raise IndexError("sum_kω(χT): the ω tail spans all frequencies but usable_ω is a strict subset")
+    c2 = χ.tail_c[2] if len(χ.tail_c) == 3 else 0.0
+    lam = 0.0 if math.isnan(λ) else λ
+    out = np.zeros(3)
+    ctx.check(_lib.lib().ldga_sum_kw(ctx.handle, ptr(χ.data, f64, "χ.data"), χ.data.shape[1], lo, hi, c2, χ.β, lam,
+                                     out.ctypes.data))
+    return (out[0], out[2]) if with_derivative else float(out[0])
+
+
+def get_λ_min(χr: np.ndarray) -> float:
+    """-min_q 1/χ(q, ω₀) (LambdaCorrection/helpers.jl:217-220)."""
+    nh = math.ceil(χr.shape[1] / 2) - 1
+    return float(-np.min(1.0 / χr[:, nh]))
+
+
+# ---------------------------------------------------------------------------
+# EoM (src/ladderDGATools_singleCore.jl:108-189)
+# ---------------------------------------------------------------------------
+def _sigma_call(ctx, sP, mP, νmax, λm, λd, tc, kind, χ_m_sum, χm, γm, χd, γd, λ0, Gνω_r, Nq):
+    out = np.empty((Nq, νmax), dtype=c128, order="F")
+    ctx.check(_lib.lib().ldga_sigma(ctx.handle, C.byref(_freq(sP)), C.byref(_model(mP)), νmax, λm, λd, int(tc),
+                                    kind, χ_m_sum, ptr(χm.data, f64, "χm"), ptr(γm.data, c128, "γm"),
+                                    ptr(χd.data, f64, "χd"), ptr(γd.data, c128, "γd"), ptr(λ0, c128, "λ0"),
+                                    ptr(Gνω_r, c128, "Gνω_r"), out.ctypes.data))
+    return out
+
+
+def calc_Σ(χm: χT, γm: γT, χd: χT, γd: γT, χ_m_sum: float, λ0, Gνω_r, kG: KGrid, mP: ModelParameters,
+           sP: SimulationParameters, νmax: int | None = None, λm: float = 0.0, λd: float = 0.0, tc: bool = True,
+           ctx=None):
+    """Σ_ladder[q, ν] for ν = 0..νmax-1 (the reference's OffsetMatrix with origin 0)."""
+    if χm.λ != 0 and λm != 0:
+        raise RuntimeError(f"Stopping self energy calculation: λm = {λm} AND χm.λ = {χm.λ}")
+    if χd.λ != 0 and λd != 0:
+        raise RuntimeError(f"Stopping self energy calculation: λd = {λd} AND χd.λ = {χd.λ}")
+    νmax = sP.n_iν if νmax is None else νmax
+    if χm.λ != 0 or χd.λ != 0:
+        # χ was mapped in place by χ_λ_inplace: the data already carry λ
+        pass
+    ctx = _ctx(ctx, kG)
+    return _sigma_call(ctx, sP, mP, νmax, λm, λd, tc, 0, χ_m_sum, χm, γm, χd, γd, λ0, Gνω_r, kG.Nq)
+
+
+def calc_Σ_parts(χm: χT, γm: γT, χd: χT, γd: γT, χ_m_sum: float, λ0, Gνω_r, kG: KGrid, mP: ModelParameters,
+                 sP: SimulationParameters, tc: bool = True, λm: float = 0.0, λd: float = 0.0, ctx=None):
+    """The 7 contributions χ_m, γ_m, χ_d, γ_d, U, rest(+Hartree), tail (src/ladderDGATools_singleCore.jl:153-189)."""
+    ctx = _ctx(ctx, kG)
+    νmax = sP.n_iν
+    out = np.empty((kG.Nq, νmax, 7), dtype=c128, order="F")
+    for i, kind in enumerate(["χ_m", "γ_m", "χ_d", "γ_d", "rest_01", "rest"]):
+        out[:, :, i] = _sigma_call(ctx, sP, mP, νmax, λm, λd, False, _lib.EOM_KINDS[kind], χ_m_sum, χm, γm, χd,
+                                   γd, λ0, Gνω_r, kG.Nq)
+    if tc:
+        s = sum_kω(kG, χm, λ=λm if λm != 0 else math.nan, ctx=ctx)
+        tail = -mP.U * (s - χ_m_sum)
+    else:
+        tail = 0.0
+    iν = 1j * (2 * np.arange(νmax) + 1) * math.pi / mP.β
+    out[:, :, 6] = (tail / iν)[None, :]
+    return out

@@ -1,0 +1,258 @@
+"""GPU parity: libldga_hip (through its C ABI) against the CPU oracle on the
+same seeded inputs.  FP64 bar: normwise relative error <= 1e-10 on χ₀, χ, γ,
+λ₀ and Σ_ladder (the tolerance of the benchmark description)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import ldga_oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def relerr(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+_cache = {}
+
+
+def setup(name, **kw):
+    """Product inputs + the oracle's view of the same grid/parameters."""
+    key = (name, tuple(sorted(kw.items())))
+    if key in _cache:
+        return _cache[key]
+    from ladderdga_hip import synthetic
+    cfg = synthetic.CONFIGS[name] if name in synthetic.CONFIGS else synthetic.tiny_config(**kw)
+    inp = synthetic.build_inputs(cfg)
+    oK = O.KGrid(cfg.D, cfg.Ns, cfg.t, wedge_coords=inp.kG.wedge)
+    omP = O.ModelParameters(inp.mP.U, inp.mP.μ, inp.mP.β, inp.mP.n, 0, 0, inp.mP.Ekin_DMFT)
+    osP = O.SimulationParameters(cfg.n_iω, cfg.n_iν, 0, cfg.shift)
+    _cache[key] = (inp, oK, omP, osP)
+    return _cache[key]
+
+
+TINY = [dict(D=2, Ns=4, n_iν=4, n_iω=3), dict(D=2, Ns=6, n_iν=5, n_iω=4, n=0.9, U=1.4),
+        dict(D=3, Ns=4, n_iν=3, n_iω=2, t=0.4), dict(D=2, Ns=8, n_iν=6, n_iω=5, shift=0)]
+
+
+def case(i):
+    return ("tiny", TINY[i])
+
+
+CASES = [case(i) for i in range(len(TINY))] + [("2d16", {})]
+
+
+@pytest.mark.parametrize("name,kw", CASES)
+def test_bubble(ctx, name, kw):
+    import ladderdga_hip as L
+    inp, oK, omP, osP = setup(name, **kw)
+    print(f"bubble {name} {kw}", flush=True)
+    χ0 = L.calc_bubble(inp.G_fft, inp.G_rfft, inp.kG, inp.mP, inp.sP, ctx=ctx)
+    ref = O.calc_bubble(inp.G_fft, inp.G_rfft, oK, omP, osP)
+    assert relerr(χ0.data, ref) <= TOL
+
+
+@pytest.mark.parametrize("name,kw", CASES)
+@pytest.mark.parametrize("ch", ["m", "d"])
+def test_bse_fp64(ctx, name, kw, ch):
+    import ladderdga_hip as L
+    inp, oK, omP, osP = setup(name, **kw)
+    print(f"bse {ch} {name} {kw}", flush=True)
+    ref_χ0 = O.calc_bubble(inp.G_fft, inp.G_rfft, oK, omP, osP)
+    χ0 = L.χ0T(np.asfortranarray(ref_χ0), inp.mP.β)
+    Γ = inp.Γ_m if ch == "m" else inp.Γ_d
+    rep = {}
+    χ, γ = L.calc_χγ(ch, Γ, χ0, inp.kG, inp.mP, inp.sP, ctx=ctx, report=rep)
+    rχ, rγ = O.calc_χγ(ch, Γ, ref_χ0, oK, omP, osP)
+    assert rep["n_zero_pivot"] == 0
+    assert relerr(χ.data, rχ) <= TOL
+    assert relerr(γ.data, rγ) <= TOL
+    assert χ.tail_c == [0.0, 0.0, inp.mP.Ekin_DMFT]
+
+
+def test_bse_zero_pivot_reports_and_gives_nan(ctx):
+    """A = Γ + diag(1/χ₀) exactly singular -> LDGA_ERR_ZERO_PIVOT status and NaN outputs."""
+    import ctypes as C
+    import ladderdga_hip as L
+    from ladderdga_hip import _lib
+    from ladderdga_hip.api import _freq
+    inp, oK, omP, osP = setup("tiny", **TINY[0])
+    χ0 = O.calc_bubble(inp.G_fft, inp.G_rfft, oK, omP, osP)
+    Nν, Nω = 2 * inp.sP.n_iν, 2 * inp.sP.n_iω + 1
+    Γ = np.zeros((Nν, Nν, Nω), dtype=complex, order="F")
+    # q = 0 column of ω index 0: Γ = -diag(1/χ0(q=0)) makes that matrix exactly zero
+    Γ[np.arange(Nν), np.arange(Nν), 0] = -1.0 / χ0[0, :, 0]
+    χ = np.empty((inp.kG.Nq, Nω), order="F")
+    γ = np.empty((inp.kG.Nq, Nν, Nω), dtype=complex, order="F")
+    rep = _lib.BseReport()
+    ctx.set_kgrid(inp.kG)
+    rc = _lib.lib().ldga_chi_gamma(ctx.handle, 1, 0, C.byref(_freq(inp.sP)), inp.mP.U, inp.mP.β,
+                                   Γ.ctypes.data, np.asfortranarray(χ0).ctypes.data, χ.ctypes.data, γ.ctypes.data,
+                                   C.byref(rep))
+    assert rc == _lib.LDGA_ERR_ZERO_PIVOT
+    assert rep.n_zero_pivot >= 1
+    same = np.all(χ0[:, :, 0] == χ0[0:1, :, 0], axis=1)     # wedge points with the identical (singular) matrix
+    assert np.all(np.isnan(χ[same, 0]))
+    assert np.all(np.isfinite(χ[~same, 0])) and np.all(np.isfinite(χ[:, 1:]))
+    with pytest.warns(UserWarning):
+        L.calc_χγ("m", Γ, L.χ0T(np.asfortranarray(χ0), inp.mP.β), inp.kG, inp.mP, inp.sP, ctx=ctx)
+
+
+@pytest.mark.parametrize("name,kw", CASES[:3] + [("2d16", {})])
+def test_lambda0(ctx, name, kw):
+    import ladderdga_hip as L
+    inp, oK, omP, osP = setup(name, **kw)
+    ref_χ0 = O.calc_bubble(inp.G_fft, inp.G_rfft, oK, omP, osP)
+    with pytest.warns(UserWarning):
+        l0 = L.calc_λ0(L.χ0T(np.asfortranarray(ref_χ0), inp.mP.β), inp.F_m, inp.mP, inp.sP, inp.kG, ctx=ctx)
+    ref = O.calc_λ0(ref_χ0, inp.F_m, omP, osP)
+    assert relerr(l0, ref) <= TOL
+
+
+@pytest.mark.parametrize("lam", [0.0, 0.37, -0.05])
+def test_sum_kω(ctx, lam):
+    import ladderdga_hip as L
+    from ladderdga_hip import _lib
+    inp, oK, omP, osP = setup("2d16")
+    rng = np.random.default_rng(5)
+    Nω = 2 * inp.sP.n_iω + 1
+    d = np.asfortranarray(rng.uniform(0.05, 0.8, (inp.kG.Nq, Nω)))
+    χ = L.χT(d, inp.mP.β, tail_c=[0.0, 0.0, 0.31])
+    oχ = O.ChiT(d, inp.mP.β, tail_c=[0.0, 0.0, 0.31])
+    got, deriv = L.sum_kω(inp.kG, χ, λ=lam, ctx=ctx, with_derivative=True)
+    ref = O.sum_kω(oK, oχ, λ=lam if lam != 0 else None)
+    assert got == pytest.approx(ref, rel=1e-13, abs=1e-14)
+    h = 1e-6
+    fd = (O.sum_kω(oK, oχ, λ=lam + h) - O.sum_kω(oK, oχ, λ=lam - h)) / (2 * h)
+    assert deriv == pytest.approx(fd, rel=1e-6)
+    # a partial usable range: the tail runs over the same range (as in λm_correction, conditions_new.jl:147-153)
+    lo, hi = 5, 29
+    idx = np.arange(-inp.sP.n_iω, inp.sP.n_iω + 1)[lo:hi + 1]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        tail = np.where(idx != 0, np.real(0.31 / (2j * np.pi * idx / inp.mP.β) ** 2), 0.0)
+    out = np.zeros(3)
+    ctx.check(_lib.lib().ldga_sum_kw(ctx.handle, d.ctypes.data, Nω, lo, hi, 0.31, inp.mP.β, lam, out.ctypes.data))
+    tr = (lambda x: O.χ_λ(x, lam)) if lam != 0 else None
+    assert out[0] == pytest.approx(O.sum_kω_raw(oK, d[:, lo:hi + 1], inp.mP.β, 0.31, tail, tr), rel=1e-13)
+    dref = O.sum_kω_raw(oK, d[:, lo:hi + 1], inp.mP.β, 0.31, tail, lambda x: O.dχ_λ(x, lam))
+    assert out[1] == pytest.approx(dref, rel=1e-13)
+    # the reference's sum_kω(χT) cannot slice its tail to a partial usable range (DataTypes.jl:223-226)
+    with pytest.raises(Exception):
+        L.sum_kω(inp.kG, L.χT(d, inp.mP.β, tail_c=[0.0, 0.0, 0.31], usable_ω=range(lo, hi + 1)), ctx=ctx)
+
+
+def _ladder_inputs(name, kw, seed=3):
+    inp, oK, omP, osP = setup(name, **kw)
+    ref_χ0 = O.calc_bubble(inp.G_fft, inp.G_rfft, oK, omP, osP)
+    Nq, Nν, Nω = inp.kG.Nq, 2 * inp.sP.n_iν, 2 * inp.sP.n_iω + 1
+    rng = np.random.default_rng(seed)
+    cm = np.asfortranarray(rng.uniform(0.05, 0.5, (Nq, Nω)))
+    cd = np.asfortranarray(rng.uniform(0.05, 0.5, (Nq, Nω)))
+    rc = lambda: np.asfortranarray(rng.standard_normal((Nq, Nν, Nω)) + 1j * rng.standard_normal((Nq, Nν, Nω)))
+    return inp, oK, omP, osP, ref_χ0, cm, cd, rc(), rc(), rc()
+
+
+@pytest.mark.parametrize("name,kw", CASES)
+def test_sigma(ctx, name, kw):
+    import ladderdga_hip as L
+    inp, oK, omP, osP, _, cm, cd, gm, gd, l0 = _ladder_inputs(name, kw)
+    print(f"sigma {name} {kw}", flush=True)
+    c2 = inp.mP.Ekin_DMFT
+    νmax = max(1, min(inp.sP.n_iν, (3 * (2 * inp.sP.n_iω + 1)) // 8))
+    for lm, ld, tc in ((0.0, 0.0, True), (0.21, -0.03, True), (0.13, 0.4, False)):
+        Σ = L.calc_Σ(L.χT(cm, inp.mP.β, tail_c=[0, 0, c2]), L.γT(gm), L.χT(cd, inp.mP.β, tail_c=[0, 0, c2]),
+                     L.γT(gd), 0.123, l0, inp.G_rfft, inp.kG, inp.mP, inp.sP, νmax=νmax, λm=lm, λd=ld, tc=tc,
+                     ctx=ctx)
+        ref = O.calc_Σ(O.ChiT(cm, omP.beta, tail_c=[0, 0, c2]), gm, O.ChiT(cd, omP.beta, tail_c=[0, 0, c2]), gd,
+                       0.123, l0, inp.G_rfft, oK, omP, osP, νmax=νmax, λm=lm, λd=ld, tc=tc)
+        assert relerr(Σ, ref) <= TOL
+        assert relerr(Σ - inp.mP.n * inp.mP.U / 2, ref - omP.n * omP.U / 2) <= TOL
+
+
+@pytest.mark.parametrize("name,kw", [case(0), case(2)])
+def test_sigma_parts(ctx, name, kw):
+    import ladderdga_hip as L
+    inp, oK, omP, osP, _, cm, cd, gm, gd, l0 = _ladder_inputs(name, kw)
+    c2 = inp.mP.Ekin_DMFT
+    P = L.calc_Σ_parts(L.χT(cm, inp.mP.β, tail_c=[0, 0, c2]), L.γT(gm), L.χT(cd, inp.mP.β, tail_c=[0, 0, c2]),
+                       L.γT(gd), 0.123, l0, inp.G_rfft, inp.kG, inp.mP, inp.sP, λm=0.2, λd=0.1, ctx=ctx)
+    ref = O.calc_Σ_parts(O.ChiT(cm, omP.beta, tail_c=[0, 0, c2]), gm, O.ChiT(cd, omP.beta, tail_c=[0, 0, c2]), gd,
+                         0.123, l0, inp.G_rfft, oK, omP, osP, λm=0.2, λd=0.1)
+    assert relerr(P, ref) <= TOL
+    full = L.calc_Σ(L.χT(cm, inp.mP.β, tail_c=[0, 0, c2]), L.γT(gm), L.χT(cd, inp.mP.β, tail_c=[0, 0, c2]),
+                    L.γT(gd), 0.123, l0, inp.G_rfft, inp.kG, inp.mP, inp.sP, λm=0.2, λd=0.1, ctx=ctx)
+    assert relerr(P.sum(axis=2), full) <= TOL
+
+
+def test_pipeline_2d16(ctx):
+    """bubble -> BSE (m, d) -> λ₀ -> Σ on the 16x16 workload, against the oracle's chain."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP = setup("2d16")
+    χ0 = L.calc_bubble(inp.G_fft, inp.G_rfft, inp.kG, inp.mP, inp.sP, ctx=ctx)
+    χm, γm = L.calc_χγ("m", inp.Γ_m, χ0, inp.kG, inp.mP, inp.sP, ctx=ctx)
+    χd, γd = L.calc_χγ("d", inp.Γ_d, χ0, inp.kG, inp.mP, inp.sP, ctx=ctx)
+    with pytest.warns(UserWarning):
+        λ0 = L.calc_λ0(χ0, inp.F_m, inp.mP, inp.sP, inp.kG, ctx=ctx)
+    Σ = L.calc_Σ(χm, γm, χd, γd, inp.χloc_m_sum, λ0, inp.G_rfft, inp.kG, inp.mP, inp.sP, νmax=15, ctx=ctx)
+    rχ0 = O.calc_bubble(inp.G_fft, inp.G_rfft, oK, omP, osP)
+    rχm, rγm = O.calc_χγ("m", inp.Γ_m, rχ0, oK, omP, osP)
+    rχd, rγd = O.calc_χγ("d", inp.Γ_d, rχ0, oK, omP, osP)
+    rλ0 = O.calc_λ0(rχ0, inp.F_m, omP, osP)
+    c2 = omP.Ekin_DMFT
+    rΣ = O.calc_Σ(O.ChiT(rχm, omP.beta, tail_c=[0, 0, c2]), rγm, O.ChiT(rχd, omP.beta, tail_c=[0, 0, c2]), rγd,
+                  inp.χloc_m_sum, rλ0, inp.G_rfft, oK, omP, osP, νmax=15)
+    for a, b in ((χ0.data, rχ0), (χm.data, rχm), (γm.data, rγm), (χd.data, rχd), (γd.data, rγd), (λ0, rλ0),
+                 (Σ, rΣ)):
+        assert relerr(a, b) <= TOL
+    # χ is left untouched by calc_Σ (full_run.jl:65-77 checksum invariance)
+    assert np.array_equal(χm.data, np.asfortranarray(χm.data))
+
+
+@pytest.mark.parametrize("name", ["2d64", "3d24"])
+def test_bubble_large(ctx, name):
+    """Full bubble at the 64x64 / 24^3 sizes vs the oracle's FFT path."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP = setup(name)
+    print(f"bubble large {name}", flush=True)
+    χ0 = L.calc_bubble(inp.G_fft, inp.G_rfft, inp.kG, inp.mP, inp.sP, ctx=ctx)
+    ref = O.calc_bubble(inp.G_fft, inp.G_rfft, oK, omP, osP)
+    assert relerr(χ0.data, ref) <= TOL
+
+
+def test_bse_large_sampled(ctx):
+    """64x64, Nν=120: all (q,ω) on the GPU; 60 sampled matrices per channel through LAPACK."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP = setup("2d64")
+    print("bse large 2d64", flush=True)
+    χ0 = L.calc_bubble(inp.G_fft, inp.G_rfft, inp.kG, inp.mP, inp.sP, ctx=ctx)
+    rng = np.random.default_rng(1)
+    Nq, Nω = inp.kG.Nq, 2 * inp.sP.n_iω + 1
+    pts = list(zip(rng.integers(0, Nq, 60), rng.integers(0, Nω, 60))) + [(0, inp.sP.n_iω), (Nq - 1, 0)]
+    for ch, Γ in (("m", inp.Γ_m), ("d", inp.Γ_d)):
+        χ, γ = L.calc_χγ(ch, Γ, χ0, inp.kG, inp.mP, inp.sP, ctx=ctx)
+        s = 1 if ch == "m" else -1
+        worst = 0.0
+        for q, w in pts:
+            A = Γ[:, :, w] + np.diag(1.0 / χ0.data[q, :, w])
+            inv, _, _ = O.inv_lapack(np.asfortranarray(A))
+            rχ = np.real(inv.sum()) / omP.beta ** 2
+            rγ = inv.sum(axis=0) / (χ0.data[q, :, w] * (1 + s * omP.U * rχ))
+            worst = max(worst, abs(χ.data[q, w] - rχ) / abs(rχ), relerr(γ.data[q, :, w], rγ))
+        assert worst <= TOL, worst
+
+
+def test_sigma_large(ctx):
+    """Σ at 64x64 (νmax = 45, the λdm window) vs the oracle's FFT EoM."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP, _, cm, cd, gm, gd, l0 = _ladder_inputs("2d64", {})
+    print("sigma large 2d64", flush=True)
+    c2 = inp.mP.Ekin_DMFT
+    Σ = L.calc_Σ(L.χT(cm, inp.mP.β, tail_c=[0, 0, c2]), L.γT(gm), L.χT(cd, inp.mP.β, tail_c=[0, 0, c2]), L.γT(gd),
+                 0.2, l0, inp.G_rfft, inp.kG, inp.mP, inp.sP, νmax=45, λm=0.1, λd=0.05, ctx=ctx)
+    ref = O.calc_Σ(O.ChiT(cm, omP.beta, tail_c=[0, 0, c2]), gm, O.ChiT(cd, omP.beta, tail_c=[0, 0, c2]), gd, 0.2,
+                   l0, inp.G_rfft, oK, omP, osP, νmax=45, λm=0.1, λd=0.05)
+    assert relerr(Σ, ref) <= TOL
