@@ -52,6 +52,83 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// DPP lane exchange inside a 16-lane row (double split into two dwords).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+
+// keep (c, i, v) of the better candidate: larger |.|, then smaller row index
+template <int CTRL>
+__device__ __forceinline__ void pivot_combine(double &bc, int &bi, double &bx, double &by) {
+  const double oc = dpp_f64<CTRL>(bc), ox = dpp_f64<CTRL>(bx), oy = dpp_f64<CTRL>(by);
+  const int oi = dpp_i32<CTRL>(bi);
+  const bool take = (oc > bc) || (oc == bc && oi < bi);
+  bc = take ? oc : bc;
+  bi = take ? oi : bi;
+  bx = take ? ox : bx;
+  by = take ? oy : by;
+}
+
+// Pivot of column k among the unpivoted rows, run by the 16 lanes (one DPP row)
+// that own column k; writes the multipliers l_i = B[i][k] / piv (0 for the
+// pivot row) for every row and records (p_k, piv_k).
+template <int RPT>
+__device__ __forceinline__ void bse_pivot(int k, int tr, unsigned pmask, const double2 (&col)[RPT], double2 *lrow,
+                                          int *piv_row, double2 *piv_val, unsigned long long *nzero) {
+  double bc = -1.0, bx = 0.0, by = 0.0;
+  int bi = 0x7fffffff;
+#pragma unroll
+  for (int a = 0; a < RPT; ++a) {
+    double c = fabs(col[a].x) + fabs(col[a].y);
+    if (!(c >= 0.0)) c = 0.0;  // NaN input: still pick a row, NaN propagates
+    if (!((pmask >> a) & 1u) && c > bc) {
+      bc = c;
+      bi = tr + 16 * a;
+      bx = col[a].x;
+      by = col[a].y;
+    }
+  }
+  // butterfly over the 16-lane row: quad xor 1, quad xor 2, half-row mirror, row mirror
+  pivot_combine<0xB1>(bc, bi, bx, by);
+  pivot_combine<0x4E>(bc, bi, bx, by);
+  pivot_combine<0x141>(bc, bi, bx, by);
+  pivot_combine<0x140>(bc, bi, bx, by);
+  const double2 inv = crecip(cz(bx, by));
+#pragma unroll
+  for (int a = 0; a < RPT; ++a) {
+    const int i = tr + 16 * a;
+    lrow[i] = (i == bi) ? cz(0.0, 0.0) : cmul(col[a], inv);
+  }
+  if (tr == 0) {
+    piv_row[k] = bi;
+    piv_val[k] = cz(bx, by);
+    if (bc == 0.0) atomicAdd(nzero, 1ull);
+  }
+}
+
+// B[a][b] -= l[a] * u for every row of the thread (one column slot b)
+template <int RPT, int CPT>
+__device__ __forceinline__ void bse_update_slot(int b, double (&Br)[RPT][CPT], double (&Bi)[RPT][CPT],
+                                                const double2 (&l)[RPT], double2 u) {
+#pragma unroll
+  for (int a = 0; a < RPT; ++a) {
+    double xr = Br[a][b], xi = Bi[a][b];
+    xr = fma(-l[a].x, u.x, xr);
+    xr = fma(l[a].y, u.y, xr);
+    xi = fma(-l[a].x, u.y, xi);
+    xi = fma(-l[a].y, u.x, xi);
+    Br[a][b] = xr;
+    Bi[a][b] = xi;
+  }
+}
+
 template <int RPT, int CPT, int TC>
 __global__ __launch_bounds__(16 * TC) void bse_gj_f64_kernel(int N, int Nq, int Nc, int shell,
                                                               const double2 *__restrict__ Gam,
@@ -95,86 +172,57 @@ __global__ __launch_bounds__(16 * TC) void bse_gj_f64_kernel(int N, int Nq, int 
     }
   }
 
+  if (tc == 0) {
+    double2 col[RPT];
+#pragma unroll
+    for (int a = 0; a < RPT; ++a) col[a] = cz(Br[a][0], Bi[a][0]);
+    bse_pivot<RPT>(0, tr, pmask, col, lbuf[0], piv_row, piv_val, nzero);
+  }
+
   for (int k = 0; k < N; ++k) {
-    const int kc = k % TC, bk = k / TC;
-    if (tc == kc) {
-      // ---- pivot search over the unpivoted rows of column k (16 lanes) ----
-      double bc = -1.0;
-      int bi = 0x7fffffff;
-      double2 bv = cz(0.0, 0.0);
-#pragma unroll
-      for (int a = 0; a < RPT; ++a) {
-        const double2 v = sel<CPT>(Br[a], Bi[a], bk);
-        double c = fabs(v.x) + fabs(v.y);
-        if (!(c >= 0.0)) c = 0.0;  // NaN input: still pick a row, NaN propagates
-        if (!((pmask >> a) & 1u) && c > bc) {
-          bc = c;
-          bi = tr + 16 * a;
-          bv = v;
-        }
-      }
-#pragma unroll
-      for (int off = 8; off > 0; off >>= 1) {
-        const double oc = __shfl_xor(bc, off);
-        const int oi = __shfl_xor(bi, off);
-        const double ox = __shfl_xor(bv.x, off), oy = __shfl_xor(bv.y, off);
-        if (oc > bc || (oc == bc && oi < bi)) {
-          bc = oc;
-          bi = oi;
-          bv = cz(ox, oy);
-        }
-      }
-      const double2 inv = crecip(bv);
-#pragma unroll
-      for (int a = 0; a < RPT; ++a) {
-        const int i = tr + 16 * a;
-        const double2 v = sel<CPT>(Br[a], Bi[a], bk);
-        lbuf[k & 1][i] = (i == bi) ? cz(0.0, 0.0) : cmul(v, inv);
-      }
-      if (tr == 0) {
-        piv_row[k] = bi;
-        piv_val[k] = bv;
-        if (bc == 0.0) atomicAdd(nzero, 1ull);
-      }
-    }
-    __syncthreads();
-    // ---- elimination of column k from every other row -------------------
+    __syncthreads();  // multipliers and pivot of column k are published
+    // ---- eliminate column k from every other row -------------------------
     const int p = piv_row[k];
     const int ap = p >> 4, trp = p & 15;
     if (tr == trp) {
       pmask |= 1u << ap;
+      // ap is wave-uniform: a scalar branch picks the register row, no selects
 #pragma unroll
-      for (int b = 0; b < CPT; ++b) {
-        double vr = Br[0][b], vi = Bi[0][b];
+      for (int a = 0; a < RPT; ++a) {
+        if (a == ap) {
 #pragma unroll
-        for (int a = 1; a < RPT; ++a) {
-          const bool m = (a == ap);
-          vr = m ? Br[a][b] : vr;
-          vi = m ? Bi[a][b] : vi;
-          __asm__ volatile("" : "+v"(vr), "+v"(vi));
+          for (int b = 0; b < CPT; ++b) prow[wave][tcl][b] = cz(Br[a][b], Bi[a][b]);
         }
-        prow[wave][tcl][b] = cz(vr, vi);
       }
     }
     wave_lds_sync();
-    double2 u[CPT];
+    double2 l[RPT];
 #pragma unroll
-    for (int b = 0; b < CPT; ++b) u[b] = prow[wave][tcl][b];
-#pragma unroll
-    for (int a = 0; a < RPT; ++a) {
-      const double2 l = lbuf[k & 1][tr + 16 * a];
+    for (int a = 0; a < RPT; ++a) l[a] = lbuf[k & 1][tr + 16 * a];
+    // Column slots whose every column (in this wave) is <= k are finished: their
+    // entries are never read again, so the wave skips them.
+    const int wmax = 4 * wave + 3;
+    const int kn = k + 1, bn = kn / TC;
+    const bool pivot_wave = kn < N && ((kn % TC) >> 2) == wave;
+    if (pivot_wave) {
+      // look-ahead: column k+1 first, then its pivot, then the remaining slots
+      double2 col[RPT];
 #pragma unroll
       for (int b = 0; b < CPT; ++b) {
-        if (tc + TC * b > k) {
-          double xr = Br[a][b], xi = Bi[a][b];
-          xr = fma(-l.x, u[b].x, xr);
-          xr = fma(l.y, u[b].y, xr);
-          xi = fma(-l.x, u[b].y, xi);
-          xi = fma(-l.y, u[b].x, xi);
-          Br[a][b] = xr;
-          Bi[a][b] = xi;
+        if (b == bn) {
+          bse_update_slot<RPT, CPT>(b, Br, Bi, l, prow[wave][tcl][b]);
+#pragma unroll
+          for (int a = 0; a < RPT; ++a) col[a] = cz(Br[a][b], Bi[a][b]);
         }
       }
+      if (tc == kn % TC) bse_pivot<RPT>(kn, tr, pmask, col, lbuf[kn & 1], piv_row, piv_val, nzero);
+#pragma unroll
+      for (int b = 0; b < CPT; ++b)
+        if (b != bn && wmax + TC * b > k) bse_update_slot<RPT, CPT>(b, Br, Bi, l, prow[wave][tcl][b]);
+    } else {
+#pragma unroll
+      for (int b = 0; b < CPT; ++b)
+        if (wmax + TC * b > k) bse_update_slot<RPT, CPT>(b, Br, Bi, l, prow[wave][tcl][b]);
     }
   }
 

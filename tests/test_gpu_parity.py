@@ -95,7 +95,8 @@ def test_bse_zero_pivot_reports_and_gives_nan(ctx):
                                    C.byref(rep))
     assert rc == _lib.LDGA_ERR_ZERO_PIVOT
     assert rep.n_zero_pivot >= 1
-    same = np.all(χ0[:, :, 0] == χ0[0:1, :, 0], axis=1)     # wedge points with the identical (singular) matrix
+    # matrices with an exactly zero diagonal entry 1/χ0[q] - 1/χ0[0] are singular
+    same = np.any(χ0[:, :, 0] == χ0[0:1, :, 0], axis=1)
     assert np.all(np.isnan(χ[same, 0]))
     assert np.all(np.isfinite(χ[~same, 0])) and np.all(np.isfinite(χ[:, 1:]))
     with pytest.warns(UserWarning):
