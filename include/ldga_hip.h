@@ -82,6 +82,14 @@ const char *ldga_last_error(const ldga_ctx *ctx);
 /* Synchronise the context's stream (host-array calls already do). */
 int ldga_sync(ldga_ctx *ctx);
 
+/* Device memory for callers that keep the path's arrays resident (the _dev
+ * entry points): plain hipMalloc / hipFree / synchronous copies on the
+ * context's device. */
+int ldga_malloc(ldga_ctx *ctx, uint64_t bytes, void **dptr);
+int ldga_free(ldga_ctx *ctx, void *dptr);
+int ldga_memcpy_h2d(ldga_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int ldga_memcpy_d2h(ldga_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+
 /* ---- k grid -----------------------------------------------------------
  * Replaces the KGrid of Dispersions.jl used at src/BSETools.jl:165,
  * src/ladderDGATools.jl:151-157 and DataTypes.jl:240-246 (kintegrate).
@@ -152,6 +160,46 @@ int ldga_sigma_dev(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int
                    int tc, int eom_kind, double chiloc_m_sum, const void *chi_m, const void *gamma_m,
                    const void *chi_d, const void *gamma_d, const void *lambda0, const void *G_rfft, void *sigma,
                    void *stream);
+
+
+/* ---- Matsubara shell sum: G_shell_sum (src/GFTools.jl:281-283) ----------
+ * (1/beta) sum_{n in shell} 1/(i nu_n)^2 + beta/4 for indices beyond N-1,
+ * = beta * trigamma(N + 1/2) / (2 pi^2).  Host-only, no context needed. */
+double ldga_G_shell_sum(int N, double beta);
+
+/* ---- lambda_dm residual: residual_vals / residual_f of λdm_correction ------
+ * (LambdaCorrection/conditions_new.jl:250-287).  For (lam_m, lam_d):
+ * Sigma = calc_Σ(...; λm, λd, tc) over nu = 0..numax-1, mu refit so that
+ * filling_pos(G_ladder[:, 0:numax-1]) = n (GFTools.jl:162-186; secant to
+ * |f| < 1e-14 when fit_mu != 0, else mu = mu0), E_kin/E_pot1 from calc_E
+ * (thermodynamics.jl:142-168), the chi sums sum_kω(chi_r^λr) and
+ *   res_pp   = (Σχd + Σχm)/2 - n/2 (1 - n/2)
+ *   res_epot = E_pot1 - [(U/2)(Σχd - Σχm) + U n^2/4].
+ * want_jac != 0 also returns d(res_pp, res_epot)/d(lam_m, lam_d) from the
+ * hand-written adjoint (implicit differentiation through the mu fit and a
+ * transposed EoM convolution); it has no counterpart in the reference, which
+ * uses a forward finite-difference Jacobian (LambdaCorrection/helpers.jl:177-180).
+ * Sigma_loc (nu = 0..last(fft_range), complex) is only read when G_ladder is
+ * requested.  Outputs sigma (Nq x numax) and G_ladder (Nq x n_fft) are optional. */
+typedef struct {
+  double n, mu, Ekin, Epot1, Epot2, lhs_c1, res_pp, res_epot;
+  double jac[4]; /* row-major: [dres_pp/dlam_m, dres_pp/dlam_d, dres_epot/dlam_m, dres_epot/dlam_d] */
+  int32_t mu_iters;
+  int32_t mu_converged;
+} ldga_lambdadm_out;
+
+int ldga_lambdadm_residual(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m,
+                           double lam_d, int tc, double chiloc_m_sum, int fit_mu, double mu0, const double *chi_m,
+                           const double *gamma_m, const double *chi_d, const double *gamma_d, const double *lambda0,
+                           const double *G_rfft, const double *Sigma_loc, int want_jac, ldga_lambdadm_out *out,
+                           double *sigma, double *G_ladder);
+/* Device-pointer form: inputs resident on the device; synchronises the stream
+ * to return `out` on the host; sigma / G_ladder are device pointers (nullable). */
+int ldga_lambdadm_residual_dev(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m,
+                               double lam_d, int tc, double chiloc_m_sum, int fit_mu, double mu0, const void *chi_m,
+                               const void *gamma_m, const void *chi_d, const void *gamma_d, const void *lambda0,
+                               const void *G_rfft, const void *Sigma_loc, int want_jac, ldga_lambdadm_out *out,
+                               void *sigma, void *G_ladder, void *stream);
 
 #ifdef __cplusplus
 }

@@ -68,7 +68,7 @@ int ldga_create(int device, ldga_ctx **out) {
 
 static void free_kgrid(ldga_ctx *ctx) {
   const void *p[] = {ctx->oct.oct2wedge, ctx->oct.wedge2oct, ctx->oct.octfull, ctx->oct.octw,
-                     ctx->oct.octphase, ctx->oct.cosT,      ctx->oct.kmult};
+                     ctx->oct.octphase, ctx->oct.cosT,      ctx->oct.kmult,   ctx->oct.eps};
   for (const void *q : p)
     if (q) hipFree(const_cast<void *>(q));
   ctx->oct = OctTables{};
@@ -91,6 +91,38 @@ const char *ldga_last_error(const ldga_ctx *ctx) { return ctx ? ctx->err.c_str()
 
 int ldga_sync(ldga_ctx *ctx) {
   if (!ctx) return LDGA_ERR_ARG;
+  LDGA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return LDGA_OK;
+}
+
+int ldga_malloc(ldga_ctx *ctx, uint64_t bytes, void **dptr) {
+  if (!ctx || !dptr) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  hipSetDevice(ctx->device);
+  LDGA_HIP_TRY(ctx, hipMalloc(dptr, std::max<uint64_t>(bytes, 1)));
+  return LDGA_OK;
+}
+
+int ldga_free(ldga_ctx *ctx, void *dptr) {
+  if (!ctx) return LDGA_ERR_ARG;
+  hipSetDevice(ctx->device);
+  LDGA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  LDGA_HIP_TRY(ctx, hipFree(dptr));
+  return LDGA_OK;
+}
+
+int ldga_memcpy_h2d(ldga_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+  if (!ctx || !dst || !src) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  hipSetDevice(ctx->device);
+  LDGA_HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  LDGA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return LDGA_OK;
+}
+
+int ldga_memcpy_d2h(ldga_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+  if (!ctx || !dst || !src) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  hipSetDevice(ctx->device);
+  LDGA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  LDGA_HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
   LDGA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return LDGA_OK;
 }
@@ -223,6 +255,12 @@ int ldga_set_kgrid(ldga_ctx *ctx, int D, int Ns, int Nq, const int32_t *wedge, c
   T.cosT = (double *)p;
   if ((rc = up(kmult.data(), Nq * 8, &p))) return rc;
   T.kmult = (double *)p;
+  {
+    std::vector<double> e(Nq, 0.0);
+    if (eps) e.assign(eps, eps + Nq);
+    if ((rc = up(e.data(), Nq * 8, &p))) return rc;
+    T.eps = (double *)p;
+  }
   T.D = D;
   T.Ns = Ns;
   T.n = n;

@@ -288,6 +288,120 @@ __global__ __launch_bounds__(kSumNT) void sum_kw_kernel(const double *__restrict
   }
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the EoM with respect to (lam_m, lam_d).  Given the weight W(k,nu)
+// = dR/dSigma(k,nu) (a symmetric function on the full grid, stored on the
+// wedge), sum_k W(k) conv(K', G)(k) = sum_q K'(q) conv(W, G)(q), so one
+// transposed convolution T = conv(W(nu), G(nu+w)) per (nu, w) contracts with
+//   dK/dlam_m = -1.5 U^2 gamma_m (chi_m^lam)^2,  dK/dlam_d = -0.5 U^2 gamma_d (chi_d^lam)^2
+// (the derivatives of eom, src/ladderDGATools.jl:260, through chi/(lam chi + 1)).
+// Per workgroup: one nu, an omega chunk; partial sums of sum_q kMult K' T.
+// ---------------------------------------------------------------------------
+template <int MAXPT>
+__global__ __launch_bounds__(kNT) void eom_adjoint_kernel(OctTables T, int Nw, int Nnu, int n_inu, int n_iw, int shift,
+                                                          int fft_first, int numax, int wchunk, double U, double lam_m,
+                                                          double lam_d, const double *__restrict__ chim,
+                                                          const double *__restrict__ chid,
+                                                          const double2 *__restrict__ gm,
+                                                          const double2 *__restrict__ gd,
+                                                          const double2 *__restrict__ Grfft,
+                                                          const double2 *__restrict__ W, double scale,
+                                                          double2 *__restrict__ partial) {
+  extern __shared__ double2 smem[];
+  double2 *buf = smem;
+  double *cs = reinterpret_cast<double *>(smem + T.NO);
+  __shared__ double2 red[2][kNT / 64];
+  const int nuii = blockIdx.x % numax;
+  const int chunk = blockIdx.x / numax;
+  for (int m = threadIdx.x; m < T.Ns; m += kNT) cs[m] = T.cosT[m];
+  // forward transform of W(., nu) (weights w(q) of the octant fold)
+  for (int o = threadIdx.x; o < T.NO; o += kNT)
+    buf[o] = cscale(W[T.oct2wedge[o] + (size_t)T.Nq * nuii], T.octw[o]);
+  __syncthreads();
+  for (int ax = 0; ax < T.D; ++ax) oct_pass<MAXPT>(buf, cs, ax, T);
+  double2 What[MAXPT];
+#pragma unroll
+  for (int p = 0; p < MAXPT; ++p) {
+    const int o = threadIdx.x + p * kNT;
+    What[p] = (o < T.NO) ? buf[o] : cz(0.0, 0.0);
+  }
+  double2 am = cz(0.0, 0.0), ad = cz(0.0, 0.0);
+  const int w0 = chunk * wchunk, w1 = min(Nw, w0 + wchunk);
+  const size_t fs = (size_t)T.Nk;
+  for (int wi = w0; wi < w1; ++wi) {
+    const int wn = wi - n_iw;
+    const int nui = n_inu + shift * trunc_half(wn) + nuii;
+    if (nui > 2 * n_inu - 1) continue;
+    const double2 *g = Grfft + (size_t)(nuii + wn - fft_first) * fs;
+    __syncthreads();  // previous users of buf are done
+#pragma unroll
+    for (int p = 0; p < MAXPT; ++p) {
+      const int o = threadIdx.x + p * kNT;
+      if (o < T.NO) buf[o] = cscale(cmul(What[p], cmul(g[T.octfull[o]], T.octphase[o])), T.octw[o]);
+    }
+    __syncthreads();
+    for (int ax = 0; ax < T.D - 1; ++ax) oct_pass<MAXPT>(buf, cs, ax, T);
+    for (int q = threadIdx.x; q < T.Nq; q += kNT) {
+      const double2 t = cscale(oct_last_at(buf, cs, q, T), scale);
+      double cm = chim[q + (size_t)T.Nq * wi], cd = chid[q + (size_t)T.Nq * wi];
+      if (lam_m != 0.0) cm = cm / (lam_m * cm + 1.0);
+      if (lam_d != 0.0) cd = cd / (lam_d * cd + 1.0);
+      const size_t idx = q + (size_t)T.Nq * (nui + (size_t)Nnu * wi);
+      const double km = T.kmult[q];
+      am = cadd(am, cscale(cmul(gm[idx], t), -1.5 * U * U * cm * cm * km));
+      ad = cadd(ad, cscale(cmul(gd[idx], t), -0.5 * U * U * cd * cd * km));
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    am.x += __shfl_xor(am.x, off);
+    am.y += __shfl_xor(am.y, off);
+    ad.x += __shfl_xor(ad.x, off);
+    ad.y += __shfl_xor(ad.y, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = am;
+    red[1][threadIdx.x >> 6] = ad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double2 sm = cz(0.0, 0.0), sd = cz(0.0, 0.0);
+    for (int i = 0; i < kNT / 64; ++i) {
+      sm = cadd(sm, red[0][i]);
+      sd = cadd(sd, red[1][i]);
+    }
+    partial[2 * blockIdx.x] = sm;
+    partial[2 * blockIdx.x + 1] = sd;
+  }
+}
+
+__global__ void sum_pairs_kernel(int n, const double2 *__restrict__ partial, double2 *__restrict__ out) {
+  __shared__ double2 red[2][4];
+  double2 a = cz(0.0, 0.0), b = cz(0.0, 0.0);
+  for (int i = threadIdx.x; i < n; i += 256) {
+    a = cadd(a, partial[2 * i]);
+    b = cadd(b, partial[2 * i + 1]);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a.x += __shfl_xor(a.x, off);
+    a.y += __shfl_xor(a.y, off);
+    b.x += __shfl_xor(b.x, off);
+    b.y += __shfl_xor(b.y, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = a;
+    red[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < 4; ++i) {
+      red[0][0] = cadd(red[0][0], red[0][i]);
+      red[1][0] = cadd(red[1][0], red[1][i]);
+    }
+    out[0] = red[0][0];
+    out[1] = red[1][0];
+  }
+}
+
 }  // namespace
 
 // number of register slots per thread needed for the octant (NO / 256 rounded up)
@@ -363,6 +477,13 @@ int launch_sum_kw(ldga_ctx *ctx, const double *chi, int Nw, int w_lo, int w_hi, 
   return LDGA_OK;
 }
 
+// omega chunking of the EoM: enough (nu, chunk) workgroups to fill the device
+int eom_chunks(int Nw, int numax, int *wchunk) {
+  int nchunks = std::max(1, std::min(Nw, (2048 + numax - 1) / numax));
+  *wchunk = (Nw + nchunks - 1) / nchunks;
+  return (Nw + *wchunk - 1) / *wchunk;
+}
+
 int launch_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m, double lam_d,
                  int tc, int eom_kind, double chiloc_m_sum, const double *chi_m, const double2 *gamma_m,
                  const double *chi_d, const double2 *gamma_d, const double2 *lambda0, const double2 *Grfft,
@@ -377,10 +498,8 @@ int launch_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int n
   // the gamma window must contain nu_n = 0 at every w (helpers.jl:58)
   if (fr->n_inu - fr->shift * (fr->n_iw / 2) < 0)
     return ldga_fail(ctx, LDGA_ERR_SHAPE, "nu window does not contain nu_n = 0 at every w");
-  // omega chunking: enough workgroups to fill the device
-  int nchunks = std::max(1, std::min(Nw, (2048 + numax - 1) / numax));
-  const int wchunk = (Nw + nchunks - 1) / nchunks;
-  nchunks = (Nw + wchunk - 1) / wchunk;
+  int wchunk = 1;
+  const int nchunks = eom_chunks(Nw, numax, &wchunk);
   double2 *partial = (double2 *)ldga_buf(ctx, "eom_partial", (size_t)T.NO * numax * nchunks * 16);
   double *tsum = (double *)ldga_buf(ctx, "eom_tail", 3 * 8);
   if (!partial || !tsum) return LDGA_ERR_HIP;
@@ -397,5 +516,26 @@ int launch_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int n
   const double scale = 1.0 / (mp->beta * T.Nk * T.Nk);
   OCT_DISPATCH(T.NO, smem, eom_final_kernel, numax, st, T, numax, nchunks, partial, scale, mp->U, mp->beta, hartree,
                add_tail ? 1 : 0, tsum, chiloc_m_sum, sigma);
+  return LDGA_OK;
+}
+
+// Adjoint contraction sum_{nu,w,q} kMult K'_r T / beta for r = m, d into out_dev[0..1] (complex).
+int launch_eom_adjoint(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m,
+                       double lam_d, const double *chi_m, const double2 *gamma_m, const double *chi_d,
+                       const double2 *gamma_d, const double2 *Grfft, const double2 *W, double2 *out_dev,
+                       hipStream_t st) {
+  const OctTables &T = ctx->oct;
+  const int Nnu = 2 * fr->n_inu, Nw = 2 * fr->n_iw + 1;
+  int wchunk = 1;
+  const int nchunks = eom_chunks(Nw, numax, &wchunk);
+  double2 *partial = (double2 *)ldga_buf(ctx, "adj_partial", (size_t)2 * numax * nchunks * 16);
+  if (!partial) return LDGA_ERR_HIP;
+  const size_t smem = (size_t)T.NO * 16 + (size_t)T.Ns * 8;
+  const double scale = 1.0 / (T.Nk * T.Nk);
+  OCT_DISPATCH(T.NO, smem, eom_adjoint_kernel, numax * nchunks, st, T, Nw, Nnu, fr->n_inu, fr->n_iw, fr->shift,
+               fr->fft_first, numax, wchunk, mp->U, lam_m, lam_d, chi_m, chi_d, gamma_m, gamma_d, Grfft, W, scale,
+               partial);
+  sum_pairs_kernel<<<1, 256, 0, st>>>(numax * nchunks, partial, out_dev);
+  LDGA_HIP_TRY(ctx, hipGetLastError());
   return LDGA_OK;
 }

@@ -40,6 +40,7 @@ struct OctTables {
   const double2 *octphase;   // NO: exp(-2 pi i sum_d R_d / Ns), undoes the reverse() of G_rfft
   const double *cosT;        // Ns: cos(2 pi m / Ns)
   const double *kmult;       // Nq: wedge multiplicities
+  const double *eps;         // Nq: dispersion on the wedge
   double Nk;                 // Ns^D
   int stride[3];             // octant strides 1, n, n^2
 };
@@ -86,7 +87,13 @@ int launch_lambda0(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double
                    double2 *lambda0, hipStream_t st);
 int launch_sum_kw(ldga_ctx *ctx, const double *chi, int Nw, int w_lo, int w_hi, double c2, double beta, double lam,
                   double *out_dev, hipStream_t st);
+// EoM partial sums: one (nu, omega-chunk) workgroup each; returns the chunking used
+int eom_chunks(int Nw, int numax, int *wchunk);
 int launch_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m, double lam_d,
                  int tc, int eom_kind, double chiloc_m_sum, const double *chi_m, const double2 *gamma_m,
                  const double *chi_d, const double2 *gamma_d, const double2 *lambda0, const double2 *Grfft,
                  double2 *sigma, hipStream_t st);
+int launch_eom_adjoint(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m,
+                       double lam_d, const double *chi_m, const double2 *gamma_m, const double *chi_d,
+                       const double2 *gamma_d, const double2 *Grfft, const double2 *W, double2 *out_dev,
+                       hipStream_t st);

@@ -34,6 +34,8 @@ EXPORTED = [
     "ldga_lambda0", "ldga_lambda0_dev",
     "ldga_sum_kw", "ldga_sum_kw_dev",
     "ldga_sigma", "ldga_sigma_dev",
+    "ldga_malloc", "ldga_free", "ldga_memcpy_h2d", "ldga_memcpy_d2h",
+    "ldga_G_shell_sum", "ldga_lambdadm_residual", "ldga_lambdadm_residual_dev",
 ]
 
 
@@ -50,6 +52,12 @@ class Model(C.Structure):
 class BseReport(C.Structure):
     _fields_ = [("n_matrices", C.c_int64), ("n_zero_pivot", C.c_int64), ("ir_steps_max", C.c_int32),
                 ("n_fallback", C.c_int32), ("max_rel_resid", C.c_double)]
+
+
+class LambdaDmOut(C.Structure):
+    _fields_ = [("n", C.c_double), ("mu", C.c_double), ("Ekin", C.c_double), ("Epot1", C.c_double),
+                ("Epot2", C.c_double), ("lhs_c1", C.c_double), ("res_pp", C.c_double), ("res_epot", C.c_double),
+                ("jac", C.c_double * 4), ("mu_iters", C.c_int32), ("mu_converged", C.c_int32)]
 
 
 class LdgaError(RuntimeError):
@@ -91,6 +99,16 @@ def lib():
                                  vp, vp, vp, vp, vp, vp, vp]),
             "ldga_sigma_dev": (i32, [vp, C.POINTER(Freq), C.POINTER(Model), i32, dbl, dbl, i32, i32, dbl,
                                      vp, vp, vp, vp, vp, vp, vp, vp]),
+            "ldga_malloc": (i32, [vp, C.c_uint64, C.POINTER(vp)]),
+            "ldga_free": (i32, [vp, vp]),
+            "ldga_memcpy_h2d": (i32, [vp, vp, vp, C.c_uint64]),
+            "ldga_memcpy_d2h": (i32, [vp, vp, vp, C.c_uint64]),
+            "ldga_G_shell_sum": (dbl, [i32, dbl]),
+            "ldga_lambdadm_residual": (i32, [vp, C.POINTER(Freq), C.POINTER(Model), i32, dbl, dbl, i32, dbl, i32,
+                                             dbl, vp, vp, vp, vp, vp, vp, vp, i32, C.POINTER(LambdaDmOut), vp, vp]),
+            "ldga_lambdadm_residual_dev": (i32, [vp, C.POINTER(Freq), C.POINTER(Model), i32, dbl, dbl, i32, dbl,
+                                                 i32, dbl, vp, vp, vp, vp, vp, vp, vp, i32, C.POINTER(LambdaDmOut),
+                                                 vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -138,6 +156,16 @@ class Context:
     def sync(self):
         self.check(lib().ldga_sync(self._h))
 
+    # resident device arrays ------------------------------------------------
+    def to_device(self, a: np.ndarray) -> "DeviceArray":
+        a = np.asfortranarray(a)
+        d = DeviceArray(self, a.nbytes, a.shape, a.dtype)
+        self.check(lib().ldga_memcpy_h2d(self._h, d.ptr, a.ctypes.data, a.nbytes))
+        return d
+
+    def empty(self, shape, dtype) -> "DeviceArray":
+        return DeviceArray(self, int(np.prod(shape)) * np.dtype(dtype).itemsize, shape, dtype)
+
     def close(self):
         if self._h:
             lib().ldga_destroy(self._h)
@@ -146,6 +174,36 @@ class Context:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A device buffer of the context with a (Fortran-order) shape and dtype."""
+
+    def __init__(self, ctx: Context, nbytes: int, shape, dtype):
+        self.ctx, self.nbytes, self.shape, self.dtype = ctx, nbytes, tuple(shape), np.dtype(dtype)
+        p = C.c_void_p()
+        ctx.check(lib().ldga_malloc(ctx.handle, nbytes, C.byref(p)))
+        self._p = p
+
+    @property
+    def ptr(self):
+        return self._p.value
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype, order="F")
+        self.ctx.check(lib().ldga_memcpy_d2h(self.ctx.handle, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self._p and self._p.value:
+            lib().ldga_free(self.ctx.handle, self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
         except Exception:
             pass
 

@@ -1,0 +1,261 @@
+"""λ corrections (LambdaCorrection/ of the reference) over libldga_hip.
+
+Host logic only: the root finders (newton_right, LambdaCorrection/helpers.jl:
+133-208) and the bookkeeping of λm_correction / λdm_correction
+(conditions_new.jl:144-324).  Every evaluation of a sum or a residual runs on
+the GPU; for λdm the ladder arrays are uploaded once and stay resident.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import LambdaDmOut, default_context
+from .api import _freq, _model, get_λ_min
+from .kgrid import KGrid
+from .types import ModelParameters, SimulationParameters, χT, γT
+
+
+@dataclass
+class λ_result:
+    """conditions_new.jl:13-46."""
+    λm: float
+    λd: float
+    type: str
+    converged: bool
+    sc_converged: bool = True
+    EKin: float = math.nan
+    EPot_p1: float = math.nan
+    EPot_p2: float = math.nan
+    PP_p1: float = math.nan
+    PP_p2: float = math.nan
+    trace: list = field(default_factory=list)
+    G_ladder: np.ndarray = None
+    Σ_ladder: np.ndarray = None
+    μ: float = math.nan
+    n: float = math.nan
+    newton_steps: int = 0
+    residual_evals: int = 0
+
+
+# ---------------------------------------------------------------------------
+# root finders (LambdaCorrection/helpers.jl:133-208)
+# ---------------------------------------------------------------------------
+def newton_right_1d(f, df, start: float, min_: float, nsteps: int = 100, atol: float = 1e-13) -> float:
+    δ = 1e-4
+    x0 = start
+    xi = x0
+    i = 1
+    while True:
+        fi = f(xi)
+        dfii = 1.0 / df(xi)
+        xi = x0 - dfii * fi
+        if abs(fi) < atol:
+            break
+        x0 = (min_ + δ + abs(min_ - x0) / 2) if xi < min_ else xi
+        if i >= nsteps:
+            break
+        i += 1
+    return xi
+
+
+def finite_difference_jacobian(f, x, fx):
+    """Forward differences, relative step sqrt(eps) (FiniteDiff.jl's default)."""
+    x = np.asarray(x, dtype=float)
+    J = np.empty((len(fx), len(x)))
+    rel = math.sqrt(np.finfo(float).eps)
+    for j in range(len(x)):
+        h = rel * max(abs(x[j]), 1.0)
+        xp = x.copy()
+        xp[j] += h
+        J[:, j] = (np.asarray(f(xp)) - fx) / h
+    return J
+
+
+def newton_right(f, start, min_, nsteps: int = 500, atol: float = 1e-8, max_reset: int = 5,
+                 reset_backoff: float = 1.0, fj=None, verbose: bool = False):
+    """2D Newton that stays right of `min_` (helpers.jl:165-208).
+
+    `fj(x) -> (f(x), J(x))` supplies an analytic Jacobian; without it J is the
+    forward finite-difference Jacobian of the reference.  Returns (x, steps).
+    """
+    start = np.asarray(start, dtype=float)
+    min_ = np.asarray(min_, dtype=float)
+    xi = start.copy()
+    i = 1
+    reset_i = 1
+    bisection_fail = False
+    while True:
+        if fj is None:
+            fi = np.asarray(f(xi), dtype=float)
+            J = finite_difference_jacobian(f, xi, fi)
+        else:
+            fi, J = fj(xi)
+            fi = np.asarray(fi, dtype=float)
+        try:
+            dfii = np.linalg.inv(J)
+        except np.linalg.LinAlgError:
+            dfii = np.full((len(xi), len(xi)), np.nan)
+        xi_last = xi.copy()
+        xi = xi - dfii @ fi
+        if verbose:
+            print(f"i={i} : xi = {xi}, fi = {fi}. {np.linalg.norm(fi)} ?<? {atol}")
+        for l in range(len(xi)):
+            if xi[l] < min_[l]:
+                xi[l] = min_[l] + abs(xi_last[l] - min_[l]) / 2
+                if abs(xi_last[l] - min_[l]) / 2 < 1e-8:
+                    print("WARNING: Bisection step is smaller than 1e-8. This indicates faulty data or no solution.")
+                    bisection_fail = True
+        if bisection_fail or not np.all(np.isfinite(fi)):
+            bisection_fail = False
+            if reset_i >= max_reset:
+                raise RuntimeError(f"λ root-finding returned NaN after {i} iterations and {max_reset} resets. "
+                                   f"min values are {min_}")
+            reset_i += 1
+            xi = start + reset_i * reset_backoff * np.abs(min_)
+        if abs(np.linalg.norm(fi)) < atol:
+            break
+        if i >= nsteps:
+            break
+        i += 1
+    return xi, i
+
+
+# ---------------------------------------------------------------------------
+# λm (conditions_new.jl:144-172)
+# ---------------------------------------------------------------------------
+def λm_correction(χm: χT, rhs: float, kG: KGrid, mP: ModelParameters, sP: SimulationParameters,
+                  validate_threshold: float = 1e-8, ctx=None):
+    """λm from Σ_{q,ω} χm^λ = rhs by 1D Newton with the reference's derivative."""
+    ctx = ctx or default_context()
+    ctx.set_kgrid(kG)
+    λm_min = get_λ_min(χm.data)
+    u = list(χm.usable_ω)
+    lo, hi = u[0], u[-1]
+    L = _lib.lib()
+    Nω = χm.data.shape[1]
+    data = np.asfortranarray(χm.data, dtype=np.float64)
+    out = np.zeros(3)
+
+    def sums(l):
+        ctx.check(L.ldga_sum_kw(ctx.handle, data.ctypes.data, Nω, lo, hi, χm.tail_c[2], χm.β, l, out.ctypes.data))
+        return out.copy()
+
+    f = lambda l: sums(l)[0] - rhs
+    df = lambda l: sums(l)[1]         # the reference's df_c1 (transform = dχ_λ, tails included)
+    λm = newton_right_1d(f, df, 0.0, λm_min)
+    check = sums(λm)[0]
+    return λm, bool(abs(rhs - check) <= validate_threshold)
+
+
+# ---------------------------------------------------------------------------
+# λdm (conditions_new.jl:219-324)
+# ---------------------------------------------------------------------------
+def gen_νω_indices(χm: χT, χd: χT, sP: SimulationParameters):
+    """LambdaCorrection/helpers.jl:280-289 with usable_ωindices (GFFit.jl:81-84)."""
+    if sP.dbg_full_eom_omega:
+        ω = list(range(χm.data.shape[1]))
+    else:
+        ω = sorted(set(χm.usable_ω) & set(χd.usable_ω))
+    return ω, min(sP.n_iν, (3 * len(ω)) // 8)
+
+
+class LambdaDmProblem:
+    """The λdm residual with the ladder arrays resident on the device."""
+
+    def __init__(self, χm: χT, γm: γT, χd: χT, γd: γT, Σ_loc, gLoc_rfft, χloc_m_sum, λ0, kG: KGrid,
+                 mP: ModelParameters, sP: SimulationParameters, νmax: int, tc: bool = True, ctx=None):
+        self.ctx = ctx or default_context()
+        self.ctx.set_kgrid(kG)
+        self.kG, self.mP, self.sP, self.νmax, self.tc = kG, mP, sP, νmax, tc
+        self.χloc_m_sum = float(χloc_m_sum)
+        up = self.ctx.to_device
+        self.d = dict(χm=up(np.asarray(χm.data, np.float64)), χd=up(np.asarray(χd.data, np.float64)),
+                      γm=up(np.asarray(γm.data, np.complex128)), γd=up(np.asarray(γd.data, np.complex128)),
+                      λ0=up(np.asarray(λ0, np.complex128)), G=up(np.asarray(gLoc_rfft, np.complex128)),
+                      Σloc=up(np.ascontiguousarray(Σ_loc[:sP.fft_range.stop], np.complex128)))
+        self.fr, self.mo = _freq(sP), _model(mP)
+        self.μ = mP.μ
+        self.evals = 0
+
+    def eval(self, λ, want_jac=False, fit_μ=True, arrays=False):
+        d = self.d
+        out = LambdaDmOut()
+        Σ = G = None
+        if arrays:
+            Σ = self.ctx.empty((self.kG.Nq, self.νmax), np.complex128)
+            G = self.ctx.empty((self.kG.Nq, len(self.sP.fft_range)), np.complex128)
+        rc = _lib.lib().ldga_lambdadm_residual_dev(
+            self.ctx.handle, C.byref(self.fr), C.byref(self.mo), self.νmax, float(λ[0]), float(λ[1]), int(self.tc),
+            self.χloc_m_sum, int(fit_μ), self.μ, d["χm"].ptr, d["γm"].ptr, d["χd"].ptr, d["γd"].ptr, d["λ0"].ptr,
+            d["G"].ptr, d["Σloc"].ptr, int(want_jac), C.byref(out), Σ.ptr if arrays else None,
+            G.ptr if arrays else None, None)
+        self.ctx.check(rc)
+        self.evals += 1
+        if fit_μ and math.isfinite(out.mu):
+            self.μ = out.mu          # the reference's closure keeps the last μ as the next start
+        res = {k: getattr(out, k) for k in ("n", "mu", "Ekin", "Epot1", "Epot2", "lhs_c1", "res_pp", "res_epot",
+                                            "mu_iters", "mu_converged")}
+        res["residual"] = np.array([out.res_pp, out.res_epot])
+        res["jac"] = np.array(out.jac[:]).reshape(2, 2) if want_jac else None
+        if arrays:
+            res["Σ_ladder"], res["G_ladder"] = Σ.to_host(), G.to_host()
+            Σ.free()
+            G.free()
+        return res
+
+    def free(self):
+        for a in self.d.values():
+            a.free()
+
+
+def λdm_correction(χm: χT, γm: γT, χd: χT, γd: γT, Σ_loc, gLoc_rfft, χloc_m_sum, λ0, kG: KGrid,
+                   mP: ModelParameters, sP: SimulationParameters, νmax: int = -1, λ_min_δ: float = 1e-4,
+                   λ_val_only: bool = False, fit_μ: bool = True, λinit=(0.0, 0.0), validate_threshold: float = 1e-8,
+                   tc: bool = True, jacobian: str = "fd", verbose: bool = False, ctx=None):
+    """λm, λd such that the Pauli sum rule and E_pot consistency hold.
+
+    jacobian = "fd" reproduces the reference (forward finite differences of the
+    residual, helpers.jl:177-180); "adjoint" uses the analytic Jacobian.
+    """
+    if χm.λ != 0 or χd.λ != 0:
+        raise RuntimeError("λ parameter already set. Aborting λdm calculation")
+    _, νmax_d = gen_νω_indices(χm, χd, sP)
+    νmax = νmax_d if νmax < 1 else min(νmax_d, νmax)
+    prob = LambdaDmProblem(χm, γm, χd, γd, Σ_loc, gLoc_rfft, χloc_m_sum, λ0, kG, mP, sP, νmax, tc=tc, ctx=ctx)
+    rhs_c1 = mP.n / 2 * (1 - mP.n / 2)
+    f = lambda λ: prob.eval(λ, fit_μ=fit_μ)["residual"]
+
+    def fj(λ):
+        r = prob.eval(λ, want_jac=True, fit_μ=fit_μ)
+        return r["residual"], r["jac"]
+
+    λm_min, λd_min = get_λ_min(χm.data), get_λ_min(χd.data)
+    min_λ = np.array([λm_min + λ_min_δ * abs(λm_min), λd_min + λ_min_δ * abs(λd_min)])
+    try:
+        root, steps = newton_right(f, list(λinit), min_λ, verbose=verbose,
+                                   fj=fj if jacobian == "adjoint" else None)
+    except RuntimeError as e:
+        print(f"Error: {e}")
+        root, steps = np.array([math.nan, math.nan]), 0
+    if np.any(np.isnan(root)):
+        print("WARNING: No λ root was found!")
+    elif np.any(root < min_λ):
+        print(f"WARNING: λ = {root} outside region ({min_λ})!")
+    if np.all(np.isfinite(root)):
+        r = prob.eval(root, fit_μ=fit_μ, arrays=True)
+        converged = abs(rhs_c1 - r["lhs_c1"]) <= validate_threshold and abs(r["Epot1"] - r["Epot2"]) <= validate_threshold
+        prob.free()
+        if λ_val_only:
+            return root[0], root[1], converged
+        Gl = r["G_ladder"]
+        return λ_result(root[0], root[1], "dm", True, converged, r["Ekin"], r["Epot1"], r["Epot2"], rhs_c1,
+                        r["lhs_c1"], [], Gl, r["Σ_ladder"], r["mu"], r["n"], steps, prob.evals)
+    prob.free()
+    if λ_val_only:
+        return root[0], root[1], False
+    return λ_result(root[0], root[1], "dm", True, False, newton_steps=steps, residual_evals=prob.evals)

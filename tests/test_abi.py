@@ -13,7 +13,7 @@ ROOT = Path(__file__).resolve().parents[1]
 
 def header_symbols():
     text = (ROOT / "include" / "ldga_hip.h").read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(ldga_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|double|const char \*)\s*(ldga_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_library_exports_header_symbols():
@@ -77,3 +77,11 @@ def test_synthetic_local_chain_matches_oracle():
     assert np.allclose(F, inp.F_m, rtol=1e-11)
     s = O.sum_ω(O.ChiT(χm, mP.beta, tail_c=[0, 0, mP.Ekin_DMFT]))[0]
     assert s == pytest.approx(inp.χloc_m_sum, rel=1e-12)
+
+
+def test_G_shell_sum_native():
+    """ldga_G_shell_sum (the library's trigamma) against scipy's polygamma form (GFTools.jl:281-283)."""
+    from ladderdga_hip import _lib
+    for N in (1, 2, 5, 16, 46, 121, 1000):
+        for β in (1.3, 10.0, 30.0):
+            assert _lib.lib().ldga_G_shell_sum(N, β) == pytest.approx(O.G_shell_sum(N, β), rel=1e-13)

@@ -1,0 +1,123 @@
+"""GPU λ corrections against the oracle: the λdm residual (conditions_new.jl:
+250-287), its adjoint Jacobian (against central finite differences of the
+oracle residual), the λdm and λm roots."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import ldga_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+_cache = {}
+
+TINY = dict(D=2, Ns=6, n_iν=6, n_iω=6, n=0.9, U=1.4, β=4.0)
+
+
+def ladder(name):
+    """Oracle ladder quantities (bubble -> BSE -> λ0) and the product's view of them."""
+    if name in _cache:
+        return _cache[name]
+    import ladderdga_hip as L
+    from ladderdga_hip import synthetic
+    cfg = synthetic.CONFIGS[name] if name in synthetic.CONFIGS else synthetic.tiny_config(**TINY)
+    inp = synthetic.build_inputs(cfg)
+    oK = O.KGrid(cfg.D, cfg.Ns, cfg.t, wedge_coords=inp.kG.wedge)
+    omP = O.ModelParameters(inp.mP.U, inp.mP.μ, inp.mP.β, inp.mP.n, 0, 0, inp.mP.Ekin_DMFT)
+    osP = O.SimulationParameters(cfg.n_iω, cfg.n_iν, 0, cfg.shift)
+    χ0 = O.calc_bubble(inp.G_fft, inp.G_rfft, oK, omP, osP)
+    χm, γm = O.calc_χγ("m", inp.Γ_m, χ0, oK, omP, osP)
+    χd, γd = O.calc_χγ("d", inp.Γ_d, χ0, oK, omP, osP)
+    λ0 = np.asfortranarray(O.calc_λ0(χ0, inp.F_m, omP, osP))
+    c2 = omP.Ekin_DMFT
+    o = dict(χm=O.ChiT(χm, omP.beta, tail_c=[0, 0, c2]), χd=O.ChiT(χd, omP.beta, tail_c=[0, 0, c2]), γm=γm, γd=γd,
+             λ0=λ0)
+    p = dict(χm=L.χT(np.asfortranarray(χm), inp.mP.β, tail_c=[0, 0, c2]), γm=L.γT(np.asfortranarray(γm)),
+             χd=L.χT(np.asfortranarray(χd), inp.mP.β, tail_c=[0, 0, c2]), γd=L.γT(np.asfortranarray(γd)))
+    _, νmax = O.gen_νω_indices(o["χm"], o["χd"], omP, osP)
+    _cache[name] = (inp, oK, omP, osP, o, p, νmax)
+    return _cache[name]
+
+
+def oracle_res(name, λ, μ0=None):
+    inp, oK, omP, osP, o, p, νmax = ladder(name)
+    r = O.λdm_residual_vals(np.array(λ, float), o["χm"], o["γm"], o["χd"], o["γd"], inp.Σ_loc, inp.G_rfft,
+                            inp.χloc_m_sum, o["λ0"], oK, omP, osP, νmax, omP.mu if μ0 is None else μ0)
+    rhs = omP.n / 2 * (1 - omP.n / 2)
+    return dict(n=r[0], mu=r[1], Ekin=r[2], Epot1=r[3], Epot2=r[4], lhs=r[5], Σ=r[6], G=r[7],
+                residual=np.array([r[5] - rhs, r[3] - r[4]]))
+
+
+def problem(ctx, name):
+    import ladderdga_hip as L
+    inp, oK, omP, osP, o, p, νmax = ladder(name)
+    return L.LambdaDmProblem(p["χm"], p["γm"], p["χd"], p["γd"], inp.Σ_loc, inp.G_rfft, inp.χloc_m_sum, o["λ0"],
+                             inp.kG, inp.mP, inp.sP, νmax, ctx=ctx)
+
+
+@pytest.mark.parametrize("name,λ", [("tiny", (60.0, -20.0)), ("tiny", (94.0, -26.0)), ("tiny", (150.0, 3.0)),
+                                     ("2d16", (192.0, -29.0)), ("2d16", (250.0, 10.0))])
+def test_lambdadm_residual_parity(ctx, name, λ):
+    print(f"residual {name} {λ}", flush=True)
+    prob = problem(ctx, name)
+    got = prob.eval(λ, arrays=True)
+    prob.free()
+    ref = oracle_res(name, λ)
+    for k, kk in (("n", "n"), ("mu", "mu"), ("Ekin", "Ekin"), ("Epot1", "Epot1"), ("Epot2", "Epot2"),
+                  ("lhs_c1", "lhs")):
+        assert got[k] == pytest.approx(ref[kk], rel=1e-10, abs=1e-12), k
+    assert got["mu_converged"] == 1
+    Σ, G = got["Σ_ladder"], got["G_ladder"]
+    assert np.max(np.abs(Σ - ref["Σ"])) <= 1e-10 * np.max(np.abs(ref["Σ"]))
+    assert np.max(np.abs(G - ref["G"])) <= 1e-10 * np.max(np.abs(ref["G"]))
+
+
+@pytest.mark.parametrize("name,λ", [("tiny", (94.0, -26.0)), ("tiny", (130.0, -10.0)), ("tiny", (40.0, 5.0))])
+def test_lambdadm_adjoint_jacobian(ctx, name, λ):
+    """Adjoint Jacobian vs central differences (h = 1e-6 relative) of the oracle residual."""
+    prob = problem(ctx, name)
+    got = prob.eval(λ, want_jac=True)
+    prob.free()
+    J = got["jac"]
+    Jfd = np.empty((2, 2))
+    for j in range(2):
+        h = 1e-6 * max(1.0, abs(λ[j]))
+        lp, lm = list(λ), list(λ)
+        lp[j] += h
+        lm[j] -= h
+        Jfd[:, j] = (oracle_res(name, lp)["residual"] - oracle_res(name, lm)["residual"]) / (2 * h)
+    assert np.allclose(J, Jfd, rtol=1e-5, atol=1e-7 * np.max(np.abs(Jfd))), (J, Jfd)
+
+
+def test_lambdadm_correction_roots(ctx):
+    """λdm root with the reference's FD Jacobian and with the adjoint: both satisfy the sum rules and
+    agree with the oracle's root; the adjoint needs fewer residual evaluations."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP, o, p, νmax = ladder("tiny")
+    oroot, _ = O.λdm_correction(o["χm"], o["γm"], o["χd"], o["γd"], inp.Σ_loc, inp.G_rfft, inp.χloc_m_sum, o["λ0"],
+                                oK, omP, osP)
+    res = {}
+    for jac in ("fd", "adjoint"):
+        r = L.λdm_correction(p["χm"], p["γm"], p["χd"], p["γd"], inp.Σ_loc, inp.G_rfft, inp.χloc_m_sum, o["λ0"],
+                             inp.kG, inp.mP, inp.sP, jacobian=jac, ctx=ctx)
+        assert r.converged
+        assert abs(r.PP_p1 - r.PP_p2) < 1e-8 and abs(r.EPot_p1 - r.EPot_p2) < 1e-8
+        assert np.allclose([r.λm, r.λd], oroot, rtol=1e-7, atol=1e-7)
+        res[jac] = r
+    assert res["adjoint"].residual_evals < res["fd"].residual_evals
+    # the returned Σ_ladder is the EoM at the root (full_run.jl:155-160)
+    ref = oracle_res("tiny", [res["fd"].λm, res["fd"].λd])
+    assert np.max(np.abs(res["fd"].Σ_ladder - ref["Σ"])) <= 1e-9 * np.max(np.abs(ref["Σ"]))
+    # χ is left untouched (checksum invariance, full_run.jl:65-77)
+    assert np.array_equal(p["χm"].data, np.asfortranarray(o["χm"].data))
+
+
+def test_lambdam_correction(ctx):
+    import ladderdga_hip as L
+    inp, oK, omP, osP, o, p, νmax = ladder("2d16")
+    rhs = omP.n * (1 - omP.n / 2) - O.sum_kω(oK, o["χd"])
+    ref = O.λm_correction(o["χm"], rhs, oK, omP, osP)
+    got, ok = L.λm_correction(p["χm"], rhs, inp.kG, inp.mP, inp.sP, ctx=ctx)
+    assert got == pytest.approx(ref, rel=1e-10)
+    assert ok
