@@ -27,6 +27,11 @@ for p in (ROOT, ROOT / "ladderdga.jl_amd"):
 METRIC = "BSE+EoM (q,ω)-points/s: mixed+IR vs FP64 GPU at Σ rel err ≤1e-10"
 FP64_PEAK_TFLOPS = 78.6
 
+# HBM bytes per BSE launch, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script, each
+# scaled by what the counter reads for a 1024 MiB device copy (--copy-calib): fetch x2.000, write x1.000.
+# Summary and commands: profiles/bse_fp64_gj_2d64.txt.
+MEASURED_TRAFFIC = {("2d64", "fp64"): 1527.4e6}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -38,7 +43,21 @@ def parse():
     ap.add_argument("--numax", type=int, default=-1, help="EoM νmax (default: the λdm window min(n_iν, 3Nω/8))")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-points", type=int, default=300, help="(q,ω) points in the CPU BSE sample")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (calibrates the HBM byte counters)")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain copy of a known size: reads and writes mib MiB each, three launches."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calib_bytes_read": n * 8, "copy_calib_bytes_written": n * 8}))
 
 
 def cpu_baseline(inp, numax, npoints_sample):
@@ -90,6 +109,9 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(the ω-sharded multi-GPU design is described but not launched)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calib:
+        copy_calibration(args.copy_calib)
+        return
     import numpy as np
     import torch
     import ladderdga_hip as L
@@ -171,7 +193,7 @@ def main():
                    "precision": args.precision, "points_per_step": pts},
         "roofline": {"bound": "mfma", "kernel": "bse (one channel per launch)", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": None, "avg_launch_ms": bse_ms,
+                     "traffic": MEASURED_TRAFFIC.get((args.config, args.precision)), "avg_launch_ms": bse_ms,
                      "flops_per_launch": bse_flops},
     }
     if not args.no_cpu:

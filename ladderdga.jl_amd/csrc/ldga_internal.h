@@ -28,8 +28,11 @@ __host__ __device__ inline double2 cdiv(double2 a, double2 b) { return cmul(a, c
 
 // ---------------------------------------------------------------------------
 // octant tables: the k-space convolutions run on the first octant
-// R in [0, Ns/2]^D of the real-space (FFT) grid, where every quantity of the
-// path is even; see DESIGN for the derivation.
+// R in [0, Ns/2]^D of the real-space (FFT) grid.  Every quantity of the path
+// is even in k (eps_k = -2t sum cos k_d and the lattice symmetries map each k
+// to its wedge point), so its transform is real-even and a cosine sum over the
+// octant with weights prod_d w(R_d), w = 1 at R_d in {0, Ns/2} and 2 elsewhere,
+// reproduces the full-grid FFT.
 // ---------------------------------------------------------------------------
 struct OctTables {
   int D, Ns, n, NO, Nq;
