@@ -22,6 +22,7 @@
 //     holds piv_k * y_k = b_{p_k}.
 // Partial pivoting picks the same pivot sequence as zgetrf (up to ties).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "ldga_internal.h"
@@ -268,6 +269,17 @@ int run_gj(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double
 int launch_bse_mixed(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam, const double2 *chi0,
                      double s, double U, double beta, double *chi, double2 *gamma, unsigned long long *counters,
                      hipStream_t st);
+int launch_bse_blk_f64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam,
+                       const double2 *chi0, double s, double U, double beta, double *chi, double2 *gamma,
+                       unsigned long long *counters, hipStream_t st);
+
+// The register-tile Gauss-Jordan kernel above (Nnu <= 127) is kept as the
+// second FP64 implementation; LDGA_BSE_F64=gj selects it instead of the
+// blocked MFMA kernel (bse_blk.hip).
+static bool use_register_gj() {
+  const char *e = getenv("LDGA_BSE_F64");
+  return e && strcmp(e, "gj") == 0;
+}
 
 int launch_bse(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double U, double beta, const double2 *Gam,
                const double2 *chi0, double *chi, double2 *gamma, ldga_bse_report *rep, hipStream_t st) {
@@ -278,7 +290,9 @@ int launch_bse(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double 
   if (!cnt) return LDGA_ERR_HIP;
   LDGA_HIP_TRY(ctx, hipMemsetAsync(cnt, 0, 4 * 8, st));
   int rc;
-  if (precision == LDGA_PREC_FP64) {
+  if (precision == LDGA_PREC_FP64 && !use_register_gj()) {
+    rc = launch_bse_blk_f64(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, st);
+  } else if (precision == LDGA_PREC_FP64) {
     if (N <= 47)
       rc = run_gj<3, 3, 16>(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, st);
     else if (N <= 79)

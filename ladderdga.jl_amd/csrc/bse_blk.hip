@@ -1,0 +1,597 @@
+// Batched Bethe-Salpeter solve on MFMA: blocked Gauss-Jordan inversion
+// (calc_χγ, src/BSETools.jl:192-248; replaces the per-(q,w) zgetrf+zgetri of
+// src/LapackWrapper.jl:64-72).
+//
+// For every (q,w) one workgroup inverts M = A^T, A = Gamma(w) + diag(1/chi0(q,.,w)),
+// in place, by Gauss-Jordan elimination with column pivoting and implicit
+// pivots (no data is ever swapped):
+//   step for row r: p = argmax_{c unpivoted} |Re M[r][c]| + |Im M[r][c]|
+//   (LAPACK's izamax criterion, first maximum), piv = M[r][p], then
+//     M[i][c] -= M[i][p] M[r][c] / piv,  M[i][p] /= piv       (i != r, c != p)
+//     M[r][c] = -M[r][c] / piv,          M[r][p] = 1 / piv.
+// Afterwards M[r][c] = inv(M)[p_r][row_of(c)], so the solution of A^T y = 1,
+// y = inv(M) 1, is y[p_r] = sum_c M[r][c] (a row sum of the stored matrix).
+//
+// Layout: the padded NC x NC matrix (NC = 16 NT, identity on the padding)
+// lives in MFMA accumulators.  Wave w owns rows 16w..16w+15 as NT tiles of
+// 16x16 (real and imaginary planes); lane (g = lane/16, n = lane%16) holds
+// column 16t+n of tile t and the four rows trow(g, j), j = 0..3, in element j.
+//
+// Rows are processed in panels of four: panel (j, w) = element j of every
+// lane group of wave w.  The owner wave copies the panel to LDS, reloads it
+// one column per lane (columns lane + 64u), runs the four pivot steps there
+// (64-lane DPP argmax, readlane broadcasts, no workgroup barrier), and
+// publishes W = (panel after) - E_P.  After one barrier every wave applies
+// the panel to its own rows as a rank-4 block update on MFMA,
+//     M[S,:] += M[S,P] W     (S = rows outside the panel, M[S,P] before),
+// the complex product split into four real K=4 products (4M scheme).  The
+// panel ownership rotates over the waves.
+//
+// T = double: the FP64 path (f64 16x16x4 MFMA).  T = float with iterative
+// refinement: the mixed path -- the FP32 inverse X is kept in the
+// accumulators and y is refined in FP64:  r = 1 - A^T y  (Gamma, chi0 in FP64),
+// y += X r, until |X r|_inf <= 2^-50 |y|_inf or the correction stops halving
+// (the FP64 residual's rounding level, kappa(A) 2^-53, is reached).
+#include <climits>
+#include <cstring>
+#include <type_traits>
+
+#include "ldga_internal.h"
+
+namespace {
+
+template <typename T>
+struct mfma_traits;
+template <>
+struct mfma_traits<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  typedef double2 c2;
+  static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // accumulator element j of lane group g holds tile row g + 4j (f64 16x16x4)
+  static __device__ __forceinline__ int trow(int g, int j) { return g + 4 * j; }
+};
+template <>
+struct mfma_traits<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  typedef float2 c2;
+  static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  // accumulator element j of lane group g holds tile row 4g + j (f32 16x16x4)
+  static __device__ __forceinline__ int trow(int g, int j) { return 4 * g + j; }
+};
+
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---- lane exchange helpers ------------------------------------------------
+template <int CTRL, int RM>
+__device__ __forceinline__ int dpp32(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, RM, 0xf, false);
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ float dppv(float old, float v) {
+  return __int_as_float(dpp32<CTRL, RM>(__float_as_int(old), __float_as_int(v)));
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ double dppv(double old, double v) {
+  const int lo = dpp32<CTRL, RM>(__double2loint(old), __double2loint(v));
+  const int hi = dpp32<CTRL, RM>(__double2hiint(old), __double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float rdlane(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double rdlane(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// f(std::integral_constant<int, j>) for a wave-uniform j in 0..3 (scalar branches)
+template <typename F>
+__device__ __forceinline__ void with_j(int j, F &&f) {
+  switch (j) {
+    case 0: f(std::integral_constant<int, 0>{}); break;
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    default: f(std::integral_constant<int, 3>{}); break;
+  }
+}
+
+// Tile t (wave-uniform) of the accumulators.  The empty asm in each branch
+// keeps the compiler from folding the branches into a dynamically indexed
+// array, which would move the accumulators to scratch memory.
+template <int NT, typename V>
+__device__ __forceinline__ void tile_select(const V (&Vr)[NT], const V (&Vi)[NT], int t, V &xr, V &xi) {
+  t = __builtin_amdgcn_readfirstlane(t);
+#pragma unroll
+  for (int k = 0; k < NT; ++k)
+    if (t == k) {
+      xr = Vr[k];
+      xi = Vi[k];
+      __asm__ volatile("" : "+v"(xr), "+v"(xi));
+    }
+}
+
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double fast_rcp(double x) { return __builtin_amdgcn_rcp(x); }
+
+// sum over the 16 lanes of each DPP row; every lane of the row gets the sum
+template <typename T>
+__device__ __forceinline__ T row_sum16(T v) {
+  v += dppv<0xB1, 0xf>(v, v);   // quad xor 1
+  v += dppv<0x4E, 0xf>(v, v);   // quad xor 2
+  v += dppv<0x141, 0xf>(v, v);  // half-row mirror
+  v += dppv<0x140, 0xf>(v, v);  // row mirror
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ typename mfma_traits<T>::c2 mkc(T x, T y) {
+  typename mfma_traits<T>::c2 r;
+  r.x = x;
+  r.y = y;
+  return r;
+}
+
+template <typename T, int NT>
+struct BlkShared {
+  typedef typename mfma_traits<T>::c2 c2;
+  static constexpr int NC = 16 * NT;
+  c2 W[3][4][NC];      // published panel update (triple-buffered)
+  c2 panel[4][NC];     // owner's staging of the panel rows
+  c2 C[NT][16][4];     // per wave: own rows at the four pivot columns
+  int P[3][4];         // pivot columns of the panel
+  int pcol_of_row[NC]; // pivot column chosen for each row
+  int row_of_col[NC];  // row whose pivot column is c
+  double2 y[NC];       // solution (FP64)
+  double2 dinv[NC];    // 1/chi0 on the diagonal (FP64)
+  double2 res[NC];     // refinement residual (FP64)
+  double red[2][NT];
+  int zero_pivot;
+};
+
+// ---------------------------------------------------------------------------
+// FP32 instances up to NT = 8 ask for four waves per SIMD (<= 128 VGPRs) so
+// that two workgroups share a CU and overlap their serial pivot chains.
+template <typename T, int NT>
+constexpr int blk_min_waves() { return (sizeof(T) == 4 && NT <= 8) ? 4 : 1; }
+
+template <typename T, int NT, bool REFINE>
+__global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_kernel(int N, int Nq, int Nc, int shell,
+                                                          const double2 *__restrict__ Gam,
+                                                          const double2 *__restrict__ chi0, double s, double U,
+                                                          double beta, int max_ir, double *__restrict__ chi_out,
+                                                          double2 *__restrict__ gamma_out,
+                                                          unsigned long long *__restrict__ counters) {
+  typedef mfma_traits<T> MT;
+  typedef typename MT::v4 v4;
+  typedef typename MT::c2 c2;
+  constexpr int NC = 16 * NT, NU = (NC + 63) / 64;
+  __shared__ BlkShared<T, NT> sh;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, n = lane & 15;
+  const int q = blockIdx.x % Nq, w = blockIdx.x / Nq;
+  const double2 *G = Gam + (size_t)w * N * N;
+
+  for (int i = tid; i < NC; i += 64 * NT) {
+    double2 d = cz(0.0, 0.0);
+    if (i < N) d = crecip(chi0[q + (size_t)Nq * (shell + i + (size_t)Nc * w)]);
+    sh.dinv[i] = d;
+  }
+  if (tid == 0) sh.zero_pivot = 0;
+  __syncthreads();
+
+  // ---- M = A^T into the accumulators: M[i][c] = Gamma[c, i, w] + delta_ic / chi0[i]
+  v4 Vr[NT], Vi[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int c = 16 * t + n;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int i = 16 * wave + MT::trow(g, j);
+      double2 v = cz(i == c ? 1.0 : 0.0, 0.0);
+      if (i < N && c < N) {
+        v = G[c + (size_t)N * i];
+        if (i == c) v = cadd(v, sh.dinv[i]);
+      }
+      Vr[t][j] = (T)v.x;
+      Vi[t][j] = (T)v.y;
+    }
+  }
+  unsigned pmask = 0;  // pivoted columns lane + 64u, bit u (panel layout)
+#pragma unroll
+  for (int u = 0; u < NU; ++u)
+    if (lane + 64 * u >= NC) pmask |= 1u << u;
+  int zero_seen = 0;
+
+  // ---- panels: a two-stage software pipeline -----------------------------------
+  // Panel kp = element kp / NT of the lane groups of wave kp % NT.  Its owner
+  // factors it one iteration early: during iteration kp-1 it applies W_{kp-1}
+  // to its four panel rows only (VALU), factors panel kp and publishes W_kp, and
+  // it applies W_{kp-1} to its whole tile (MFMA) at the start of iteration kp.
+  // The other waves' MFMA updates thus overlap the serial pivot chain.  W and P
+  // are triple-buffered: W_kp is read up to iteration kp+1 and rewritten by the
+  // owner of panel kp+3 after the barrier of iteration kp+2.
+  constexpr int NP = 4 * NT;
+  static_assert(NT >= 2, "the pipeline needs two waves");
+
+  // owner of panel kp1 (rows: element j1): stage its panel rows, updated by W[wb]
+  // with the A operand rows in sh.C[wave] when pre, factor them, publish W_kp1.
+  auto factor = [&](int kp1, int j1, bool pre, int wb) {
+    with_j(j1, [&](auto JJ) {
+      constexpr int J = decltype(JJ)::value;
+      T xr[NT], xi[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        xr[t] = Vr[t][J];
+        xi[t] = Vi[t][J];
+      }
+      if (pre) {
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+          const c2 cr = sh.C[wave][MT::trow(g, J)][st];
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+            const c2 bb = sh.W[wb][st][16 * t + n];
+            xr[t] = fma(cr.x, bb.x, fma(-cr.y, bb.y, xr[t]));
+            xi[t] = fma(cr.x, bb.y, fma(cr.y, bb.x, xi[t]));
+          }
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) sh.panel[g][16 * t + n] = mkc<T>(xr[t], xi[t]);
+    });
+    wave_sync_lds();
+    T pr[NU][4], pim[NU][4];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int c = lane + 64 * u;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        c2 v = mkc<T>(0, 0);
+        if (c < NC) v = sh.panel[r][c];
+        pr[u][r] = v.x;
+        pim[u][r] = v.y;
+      }
+    }
+    int P[4];
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      // argmax over the unpivoted columns of panel row st, as one 32-bit key per
+      // lane: the float bits of |re| + |im| (monotone for non-negative values)
+      // with the low byte replaced by 255 - column, so ties go to the smaller
+      // column; pivoted and padding columns have key 0.
+      unsigned key = 0;
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        float v = (float)(fabs(pr[u][st]) + fabs(pim[u][st]));
+        if (!(v >= 0.0f)) v = 0.0f;  // NaN input: still pick a column, NaN propagates
+        const unsigned k = (__float_as_uint(v) & 0xffffff00u) | (unsigned)(255 - (lane + 64 * u));
+        if (!((pmask >> u) & 1u)) key = max(key, k);
+      }
+      key = max(key, (unsigned)dpp32<0xB1, 0xf>((int)key, (int)key));
+      key = max(key, (unsigned)dpp32<0x4E, 0xf>((int)key, (int)key));
+      key = max(key, (unsigned)dpp32<0x141, 0xf>((int)key, (int)key));
+      key = max(key, (unsigned)dpp32<0x140, 0xf>((int)key, (int)key));
+      key = max(key, (unsigned)dpp32<0x142, 0xa>((int)key, (int)key));  // row_bcast:15 into rows 1, 3
+      key = max(key, (unsigned)dpp32<0x143, 0xc>((int)key, (int)key));  // row_bcast:31 into rows 2, 3
+      const int p = 255 - (int)(__builtin_amdgcn_readlane((int)key, 63) & 0xff);
+      P[st] = p;
+      const int up = p >> 6, lp = p & 63;
+      if (lane == lp) pmask |= 1u << up;
+      // the pivot column at the four panel rows (row st gives the pivot)
+      T cpx[4], cpy[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        T xr = pr[0][r], xi = pim[0][r];
+#pragma unroll
+        for (int u = 1; u < NU; ++u)
+          if (up == u) {
+            xr = pr[u][r];
+            xi = pim[u][r];
+          }
+        cpx[r] = rdlane(xr, lp);
+        cpy[r] = rdlane(xi, lp);
+      }
+      const T px = cpx[st], py = cpy[st];
+      zero_seen |= (px == (T)0 && py == (T)0);
+      // 1 / piv = conj(piv) / |piv|^2, reciprocal refined by Newton steps
+      const T dd = px * px + py * py;
+      T rd = fast_rcp(dd);
+      rd = fma(rd, fma(-dd, rd, (T)1), rd);
+      rd = fma(rd, fma(-dd, rd, (T)1), rd);
+      const T ivx = px * rd, ivy = -py * rd;
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        const T rsx = pr[u][st] * ivx - pim[u][st] * ivy;
+        const T rsy = pr[u][st] * ivy + pim[u][st] * ivx;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (r == st) continue;
+          pr[u][r] = fma(-cpx[r], rsx, fma(cpy[r], rsy, pr[u][r]));
+          pim[u][r] = fma(-cpx[r], rsy, fma(-cpy[r], rsx, pim[u][r]));
+        }
+        pr[u][st] = -rsx;
+        pim[u][st] = -rsy;
+      }
+      // pivot column: M[i][p] = M[i][p] / piv, M[r][p] = 1 / piv
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        if (u == up) {  // uniform
+          if (lane == lp) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              pr[u][r] = r == st ? ivx : cpx[r] * ivx - cpy[r] * ivy;
+              pim[u][r] = r == st ? ivy : cpx[r] * ivy + cpy[r] * ivx;
+            }
+          }
+        }
+      }
+    }
+    // publish W = panel - E_P, the pivots and the row <-> column maps
+    const int bo = kp1 % 3;
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int c = lane + 64 * u;
+      if (c < NC) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sh.W[bo][r][c] = mkc<T>(pr[u][r] - (c == P[r] ? (T)1 : (T)0), pim[u][r]);
+      }
+    }
+    if (lane < 4) {
+      const int p = lane == 0 ? P[0] : lane == 1 ? P[1] : lane == 2 ? P[2] : P[3];
+      const int row = 16 * wave + MT::trow(lane, j1);
+      sh.P[bo][lane] = p;
+      sh.pcol_of_row[row] = p;
+      sh.row_of_col[p] = row;
+    }
+  };
+
+  c2 a_def = mkc<T>(0, 0);  // A operand of the owner's deferred update
+  if (wave == 0) factor(0, 0, false, 0);
+  for (int kp = 0; kp < NP; ++kp) {
+    const int o = kp % NT, j = kp / NT, bw = kp % 3;
+    const int kp1 = kp + 1, o1 = kp1 % NT;
+    __syncthreads();  // W_kp, P_kp published
+    int Ps[4];
+#pragma unroll
+    for (int st = 0; st < 4; ++st) Ps[st] = __builtin_amdgcn_readfirstlane(sh.P[bw][st]);
+    if (wave != o) {
+#pragma unroll
+      for (int st = 0; st < 4; ++st)
+        if ((Ps[st] & 63) == lane) pmask |= 1u << (Ps[st] >> 6);
+    }
+    if (wave == o && kp > 0) {  // the update by W_{kp-1} deferred while this wave factored panel kp
+      const int bp = (kp + 2) % 3;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const c2 bb = sh.W[bp][g][16 * t + n];
+        Vr[t] = MT::mma(a_def.x, bb.x, Vr[t]);
+        Vr[t] = MT::mma(-a_def.y, bb.y, Vr[t]);
+        Vi[t] = MT::mma(a_def.x, bb.y, Vi[t]);
+        Vi[t] = MT::mma(a_def.y, bb.x, Vi[t]);
+      }
+    }
+    // own rows at the four pivot columns of panel kp (the MFMA A operand)
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      v4 xr, xi;
+      tile_select<NT>(Vr, Vi, Ps[st] >> 4, xr, xi);  // uniform tile index: scalar branches only
+      if (n == (Ps[st] & 15)) {
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) sh.C[wave][MT::trow(g, jj)][st] = mkc<T>(xr[jj], xi[jj]);
+      }
+    }
+    wave_sync_lds();
+    const c2 a = sh.C[wave][n][g];  // A operand: tile row n, k = g
+    if (kp1 < NP && wave == o1) {
+      factor(kp1, kp1 / NT, true, bw);
+      a_def = a;
+    } else {
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const c2 bb = sh.W[bw][g][16 * t + n];  // B operand: k = g, column 16t + n
+        Vr[t] = MT::mma(a.x, bb.x, Vr[t]);
+        Vr[t] = MT::mma(-a.y, bb.y, Vr[t]);
+        Vi[t] = MT::mma(a.x, bb.y, Vi[t]);
+        Vi[t] = MT::mma(a.y, bb.x, Vi[t]);
+      }
+    }
+    if (wave == o) {  // panel rows: their final values
+      const int pg = Ps[0] * (g == 0) + Ps[1] * (g == 1) + Ps[2] * (g == 2) + Ps[3] * (g == 3);
+      with_j(j, [&](auto JJ) {
+        constexpr int J = decltype(JJ)::value;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const c2 v = sh.W[bw][g][16 * t + n];
+          Vr[t][J] = v.x + ((16 * t + n) == pg ? (T)1 : (T)0);
+          Vi[t][J] = v.y;
+        }
+      });
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (zero_seen && lane == 0) sh.zero_pivot = 1;
+
+  // ---- y[p_r] = sum_c M[r][c] ------------------------------------------------
+#pragma unroll
+  for (int jj = 0; jj < 4; ++jj) {
+    T sx = 0, sy = 0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      sx += Vr[t][jj];
+      sy += Vi[t][jj];
+    }
+    sx = row_sum16(sx);
+    sy = row_sum16(sy);
+    if (n == 0) {
+      const int row = 16 * wave + MT::trow(g, jj);
+      sh.y[sh.pcol_of_row[row]] = cz((double)sx, (double)sy);
+    }
+  }
+  __syncthreads();
+
+  if constexpr (REFINE) {
+    // ---- FP64 iterative refinement with the FP32 inverse --------------------
+    int it = 0;
+    double last_ratio = 0.0;
+    for (; it < max_ir; ++it) {
+      // r_i = 1 - sum_j Gamma[j, i] y_j - y_i / chi0_i; wave w: rows 4k + g for k = w, w + NT, ...
+      for (int ib = 4 * wave; ib < N; ib += 4 * NT) {
+        const int i = ib + g;
+        double2 acc = cz(0.0, 0.0);
+        if (i < N) {
+          const double2 *col = G + (size_t)N * i;
+          for (int jx = n; jx < N; jx += 16) acc = cadd(acc, cmul(col[jx], sh.y[jx]));
+        }
+        acc.x = row_sum16(acc.x);
+        acc.y = row_sum16(acc.y);
+        if (n == 0 && i < N) {
+          const double2 yi = sh.y[i];
+          const double2 d = cmul(yi, sh.dinv[i]);
+          sh.res[i] = cz(1.0 - acc.x - d.x, -acc.y - d.y);
+        }
+      }
+      if (tid < NC - N) sh.res[N + tid] = cz(0.0, 0.0);
+      __syncthreads();
+      // dy[p_r] = sum_c X[r][c] res[row_of(c)]  (FP32 products)
+      double dmax = 0.0, ymax = 0.0;
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) {
+        T sx = 0, sy = 0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const double2 rv = sh.res[sh.row_of_col[16 * t + n]];
+          const T rx = (T)rv.x, ry = (T)rv.y;
+          sx += Vr[t][jj] * rx - Vi[t][jj] * ry;
+          sy += Vr[t][jj] * ry + Vi[t][jj] * rx;
+        }
+        sx = row_sum16(sx);
+        sy = row_sum16(sy);
+        if (n == 0) {
+          const int row = 16 * wave + MT::trow(g, jj);
+          const int pc = sh.pcol_of_row[row];
+          if (pc < N) {
+            const double2 yo = sh.y[pc];
+            const double2 yn = cz(yo.x + (double)sx, yo.y + (double)sy);
+            sh.y[pc] = yn;
+            dmax = fmax(dmax, fabs((double)sx) + fabs((double)sy));
+            ymax = fmax(ymax, fabs(yn.x) + fabs(yn.y));
+          }
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        dmax = fmax(dmax, __shfl_xor(dmax, off));
+        ymax = fmax(ymax, __shfl_xor(ymax, off));
+      }
+      if (lane == 0) {
+        sh.red[0][wave] = dmax;
+        sh.red[1][wave] = ymax;
+      }
+      __syncthreads();
+      double dm = 0.0, ym = 0.0;
+#pragma unroll
+      for (int k = 0; k < NT; ++k) {
+        dm = fmax(dm, sh.red[0][k]);
+        ym = fmax(ym, sh.red[1][k]);
+      }
+      const double ratio = dm / ym;
+      __syncthreads();  // red[] is rewritten by the next step
+      // converged, or stagnating at the FP64 residual's own rounding level
+      // (kappa(A) * 2^-53: the correction no longer halves)
+      const bool stop = !(ratio > 0x1p-50) || (it > 0 && !(ratio < 0.5 * last_ratio));
+      last_ratio = ratio;
+      if (stop) {
+        ++it;
+        break;
+      }
+    }
+    if (tid == 0) {
+      atomicMax(&counters[1], (unsigned long long)it);
+      if (!(last_ratio <= 0x1p-30)) atomicAdd(&counters[2], 1ull);
+      unsigned long long bits;
+      const double lr = last_ratio == last_ratio ? last_ratio : 1.0;
+      memcpy(&bits, &lr, 8);
+      atomicMax(&counters[3], bits);  // non-negative doubles order like their bit patterns
+    }
+  }
+
+  // ---- chi and gamma ---------------------------------------------------------
+  double part = 0.0;
+  for (int k = tid; k < N; k += 64 * NT) part += sh.y[k].x;
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+  if (lane == 0) sh.red[0][wave] = part;
+  __syncthreads();
+  double tot = 0.0;
+#pragma unroll
+  for (int k = 0; k < NT; ++k) tot += sh.red[0][k];
+  const double chi = tot / (beta * beta);
+  const double fac = 1.0 + s * U * chi;
+  for (int k = tid; k < N; k += 64 * NT) {
+    const double2 c0 = chi0[q + (size_t)Nq * (k + (size_t)Nc * w)];  // no shell offset (BSETools.jl:236)
+    gamma_out[q + (size_t)Nq * (k + (size_t)N * w)] = cdiv(sh.y[k], cscale(c0, fac));
+  }
+  if (tid == 0) {
+    chi_out[q + (size_t)Nq * w] = chi;
+    if (sh.zero_pivot) atomicAdd(&counters[0], 1ull);
+  }
+}
+
+template <typename T, int NT, bool REFINE>
+int run_blk(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam, const double2 *chi0,
+            double s, double U, double beta, int max_ir, double *chi, double2 *gamma, unsigned long long *cnt,
+            hipStream_t st) {
+  if (N > 16 * NT) return ldga_fail(ctx, LDGA_ERR_SHAPE, "BSE tile too small for N=%d", N);
+  bse_blk_kernel<T, NT, REFINE><<<dim3((unsigned)Nq * Nw), dim3(64 * NT), 0, st>>>(
+      N, Nq, Nc, shell, Gam, chi0, s, U, beta, max_ir, chi, gamma, cnt);
+  LDGA_HIP_TRY(ctx, hipGetLastError());
+  return LDGA_OK;
+}
+
+}  // namespace
+
+// FP64 blocked kernel: Nnu <= 128.
+int launch_bse_blk_f64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam,
+                       const double2 *chi0, double s, double U, double beta, double *chi, double2 *gamma,
+                       unsigned long long *cnt, hipStream_t st) {
+  const int nt = (N + 15) / 16;
+  switch (nt) {
+    case 1:
+    case 2:
+    case 3: return run_blk<double, 3, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, st);
+    case 4: return run_blk<double, 4, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, st);
+    case 5: return run_blk<double, 5, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, st);
+    case 6: return run_blk<double, 6, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, st);
+    case 7: return run_blk<double, 7, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, st);
+    case 8: return run_blk<double, 8, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, st);
+    default: return ldga_fail(ctx, LDGA_ERR_SHAPE, "FP64 blocked BSE supports Nnu <= 128 (got %d)", N);
+  }
+}
+
+// Mixed: FP32 inverse on MFMA + FP64 refinement, Nnu <= 160.
+int launch_bse_mixed(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam, const double2 *chi0,
+                     double s, double U, double beta, double *chi, double2 *gamma, unsigned long long *cnt,
+                     hipStream_t st) {
+  const int nt = (N + 15) / 16, ir = 6;
+  switch (nt) {
+    case 1:
+    case 2:
+    case 3: return run_blk<float, 3, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, st);
+    case 4: return run_blk<float, 4, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, st);
+    case 5: return run_blk<float, 5, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, st);
+    case 6: return run_blk<float, 6, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, st);
+    case 7: return run_blk<float, 7, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, st);
+    case 8: return run_blk<float, 8, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, st);
+    case 9: return run_blk<float, 9, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, st);
+    case 10: return run_blk<float, 10, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, st);
+    default: return ldga_fail(ctx, LDGA_ERR_SHAPE, "mixed BSE supports Nnu <= 160 (got %d)", N);
+  }
+}

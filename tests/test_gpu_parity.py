@@ -74,6 +74,55 @@ def test_bse_fp64(ctx, name, kw, ch):
     assert χ.tail_c == [0.0, 0.0, inp.mP.Ekin_DMFT]
 
 
+@pytest.mark.parametrize("name,kw", CASES)
+@pytest.mark.parametrize("ch", ["m", "d"])
+def test_bse_mixed(ctx, name, kw, ch):
+    """FP32 inverse on MFMA + FP64 refinement: same bar as FP64."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP = setup(name, **kw)
+    print(f"bse mixed {ch} {name} {kw}", flush=True)
+    ref_χ0 = O.calc_bubble(inp.G_fft, inp.G_rfft, oK, omP, osP)
+    χ0 = L.χ0T(np.asfortranarray(ref_χ0), inp.mP.β)
+    Γ = inp.Γ_m if ch == "m" else inp.Γ_d
+    rep = {}
+    χ, γ = L.calc_χγ(ch, Γ, χ0, inp.kG, inp.mP, inp.sP, precision="mixed", ctx=ctx, report=rep)
+    rχ, rγ = O.calc_χγ(ch, Γ, ref_χ0, oK, omP, osP)
+    assert rep["n_zero_pivot"] == 0 and rep["n_fallback"] == 0
+    assert 1 <= rep["ir_steps_max"] <= 6
+    assert relerr(χ.data, rχ) <= TOL
+    assert relerr(γ.data, rγ) <= TOL
+
+
+def test_bse_register_gj_kernel(ctx, monkeypatch):
+    """The second FP64 implementation (register-tile Gauss-Jordan, LDGA_BSE_F64=gj) agrees too."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP = setup("2d16")
+    ref_χ0 = O.calc_bubble(inp.G_fft, inp.G_rfft, oK, omP, osP)
+    χ0 = L.χ0T(np.asfortranarray(ref_χ0), inp.mP.β)
+    monkeypatch.setenv("LDGA_BSE_F64", "gj")
+    χ, γ = L.calc_χγ("m", inp.Γ_m, χ0, inp.kG, inp.mP, inp.sP, ctx=ctx)
+    monkeypatch.delenv("LDGA_BSE_F64")
+    rχ, rγ = O.calc_χγ("m", inp.Γ_m, ref_χ0, oK, omP, osP)
+    assert relerr(χ.data, rχ) <= TOL and relerr(γ.data, rγ) <= TOL
+
+
+def test_bse_mixed_nnu160(ctx):
+    """Nν = 160 (the 32^3 workload's matrix size) on a small 3D grid, mixed precision vs LAPACK."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP = setup("tiny", D=3, Ns=4, n_iν=80, n_iω=1, U=2.0, β=25.0, n=1.0, t=0.408248290463863)
+    print("bse mixed Nν=160", flush=True)
+    ref_χ0 = O.calc_bubble(inp.G_fft, inp.G_rfft, oK, omP, osP)
+    χ0 = L.χ0T(np.asfortranarray(ref_χ0), inp.mP.β)
+    for ch, Γ in (("m", inp.Γ_m), ("d", inp.Γ_d)):
+        rep = {}
+        χ, γ = L.calc_χγ(ch, Γ, χ0, inp.kG, inp.mP, inp.sP, precision="mixed", ctx=ctx, report=rep)
+        rχ, rγ = O.calc_χγ(ch, Γ, ref_χ0, oK, omP, osP)
+        assert rep["n_fallback"] == 0
+        assert relerr(χ.data, rχ) <= TOL and relerr(γ.data, rγ) <= TOL
+    with pytest.raises(L.LdgaError):
+        L.calc_χγ("m", inp.Γ_m, χ0, inp.kG, inp.mP, inp.sP, precision="fp64", ctx=ctx)
+
+
 def test_bse_zero_pivot_reports_and_gives_nan(ctx):
     """A = Γ + diag(1/χ₀) exactly singular -> LDGA_ERR_ZERO_PIVOT status and NaN outputs."""
     import ctypes as C
@@ -193,13 +242,14 @@ def test_sigma_parts(ctx, name, kw):
     assert relerr(P.sum(axis=2), full) <= TOL
 
 
-def test_pipeline_2d16(ctx):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pipeline_2d16(ctx, precision):
     """bubble -> BSE (m, d) -> λ₀ -> Σ on the 16x16 workload, against the oracle's chain."""
     import ladderdga_hip as L
     inp, oK, omP, osP = setup("2d16")
     χ0 = L.calc_bubble(inp.G_fft, inp.G_rfft, inp.kG, inp.mP, inp.sP, ctx=ctx)
-    χm, γm = L.calc_χγ("m", inp.Γ_m, χ0, inp.kG, inp.mP, inp.sP, ctx=ctx)
-    χd, γd = L.calc_χγ("d", inp.Γ_d, χ0, inp.kG, inp.mP, inp.sP, ctx=ctx)
+    χm, γm = L.calc_χγ("m", inp.Γ_m, χ0, inp.kG, inp.mP, inp.sP, precision=precision, ctx=ctx)
+    χd, γd = L.calc_χγ("d", inp.Γ_d, χ0, inp.kG, inp.mP, inp.sP, precision=precision, ctx=ctx)
     with pytest.warns(UserWarning):
         λ0 = L.calc_λ0(χ0, inp.F_m, inp.mP, inp.sP, inp.kG, ctx=ctx)
     Σ = L.calc_Σ(χm, γm, χd, γd, inp.χloc_m_sum, λ0, inp.G_rfft, inp.kG, inp.mP, inp.sP, νmax=15, ctx=ctx)
@@ -228,17 +278,20 @@ def test_bubble_large(ctx, name):
     assert relerr(χ0.data, ref) <= TOL
 
 
-def test_bse_large_sampled(ctx):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_bse_large_sampled(ctx, precision):
     """64x64, Nν=120: all (q,ω) on the GPU; 60 sampled matrices per channel through LAPACK."""
     import ladderdga_hip as L
     inp, oK, omP, osP = setup("2d64")
-    print("bse large 2d64", flush=True)
+    print(f"bse large 2d64 {precision}", flush=True)
     χ0 = L.calc_bubble(inp.G_fft, inp.G_rfft, inp.kG, inp.mP, inp.sP, ctx=ctx)
     rng = np.random.default_rng(1)
     Nq, Nω = inp.kG.Nq, 2 * inp.sP.n_iω + 1
     pts = list(zip(rng.integers(0, Nq, 60), rng.integers(0, Nω, 60))) + [(0, inp.sP.n_iω), (Nq - 1, 0)]
     for ch, Γ in (("m", inp.Γ_m), ("d", inp.Γ_d)):
-        χ, γ = L.calc_χγ(ch, Γ, χ0, inp.kG, inp.mP, inp.sP, ctx=ctx)
+        rep = {}
+        χ, γ = L.calc_χγ(ch, Γ, χ0, inp.kG, inp.mP, inp.sP, precision=precision, ctx=ctx, report=rep)
+        print(f"  {ch}: report {rep}", flush=True)
         s = 1 if ch == "m" else -1
         worst = 0.0
         for q, w in pts:
