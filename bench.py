@@ -5,6 +5,11 @@ Nν=120, Nω=121, νmax=45) with every input resident in HBM:
     bubble -> BSE (m) -> BSE (d) -> λ₀ -> EoM Σ (with the λ sums of its tail)
 Throughput is (q,ω)-points per second (Nq*Nω = 561*121 points per step).
 
+The headline path (--precision, default mixed: FP32 inversion on MFMA with
+FP64 iterative refinement) is timed first, then the other path (FP64) on the
+same inputs; the JSON carries both, their ratio and the mixed path's relative
+error against the FP64 path on Σ, χ and γ (the 1e-10 gate on Σ).
+
 Prints ONE JSON line.  The roofline entry is for the dominant kernel (the BSE),
 timed with HIP events on the stream it runs on; the CPU baseline times the
 oracle (numpy/scipy FP64, one thread) on a bounded sample of the same work.
@@ -25,7 +30,8 @@ for p in (ROOT, ROOT / "ladderdga.jl_amd"):
         sys.path.insert(0, str(p))
 
 METRIC = "BSE+EoM (q,ω)-points/s: mixed+IR vs FP64 GPU at Σ rel err ≤1e-10"
-FP64_PEAK_TFLOPS = 78.6
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (MFMA and vector)
+FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32
 
 # HBM bytes per BSE launch, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script, each
 # scaled by what the counter reads for a 1024 MiB device copy (--copy-calib): fetch x2.000, write x1.000.
@@ -39,7 +45,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="2d64")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"],
+                    help="the headline path; the other one is timed beside it")
     ap.add_argument("--numax", type=int, default=-1, help="EoM νmax (default: the λdm window min(n_iν, 3Nω/8))")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-points", type=int, default=300, help="(q,ω) points in the CPU BSE sample")
@@ -112,6 +119,8 @@ def main():
     if args.copy_calib:
         copy_calibration(args.copy_calib)
         return
+    import ctypes as C
+
     import numpy as np
     import torch
     import ladderdga_hip as L
@@ -122,6 +131,8 @@ def main():
     inp = synthetic.build_inputs(cfg)
     kG, mP, sP = inp.kG, inp.mP, inp.sP
     Nq, Nν, Nω = kG.Nq, 2 * cfg.n_iν, 2 * cfg.n_iω + 1
+    Nk = cfg.Ns ** cfg.D
+    Nfft = inp.G_fft.shape[-1]
     numax = args.numax if args.numax > 0 else min(cfg.n_iν, (3 * Nω) // 8)
     dev = torch.device("cuda:0")
 
@@ -132,69 +143,100 @@ def main():
     Γm, Γd, Fm = up(inp.Γ_m), up(inp.Γ_d), up(inp.F_m)
     cplx = torch.complex128
     χ0 = torch.empty(Nq * Nν * Nω, dtype=cplx, device=dev)
-    χm = torch.empty(Nq * Nω, dtype=torch.float64, device=dev)
-    χd = torch.empty_like(χm)
-    γm = torch.empty(Nq * Nν * Nω, dtype=cplx, device=dev)
-    γd, λ0 = torch.empty_like(γm), torch.empty_like(γm)
-    Σ = torch.empty(Nq * numax, dtype=cplx, device=dev)
+    out = {}
+    for prec in ("mixed", "fp64"):
+        out[prec] = dict(χm=torch.empty(Nq * Nω, dtype=torch.float64, device=dev),
+                         χd=torch.empty(Nq * Nω, dtype=torch.float64, device=dev),
+                         γm=torch.empty(Nq * Nν * Nω, dtype=cplx, device=dev),
+                         γd=torch.empty(Nq * Nν * Nω, dtype=cplx, device=dev),
+                         Σ=torch.empty(Nq * numax, dtype=cplx, device=dev))
+    λ0 = torch.empty(Nq * Nν * Nω, dtype=cplx, device=dev)
 
     ctx = L.Context(0)
     ctx.set_kgrid(kG)
     Lb = _lib.lib()
     fr, mo = _freq(sP), _model(mP)
-    prec = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}[args.precision]
     # a dedicated stream (default priority): the library treats a NULL stream as its own
     stream = torch.cuda.Stream(device=dev)
     torch.cuda.set_stream(stream)
     sp = stream.cuda_stream
-    import ctypes as C
     frp, mop = C.byref(fr), C.byref(mo)
-    ev = []
+    PREC = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}
 
-    def step(record):
+    def ev():
+        return torch.cuda.Event(enable_timing=True)
+
+    def step(prec, rec):
+        o = out[prec]
+        e = [ev() for _ in range(5)] if rec is not None else None
+        if e: e[0].record(stream)
         ctx.check(Lb.ldga_bubble_dev(ctx.handle, frp, mP.β, Gf.data_ptr(), Gr.data_ptr(), χ0.data_ptr(), sp))
-        for s, Γ, χ, γ in ((1, Γm, χm, γm), (-1, Γd, χd, γd)):
-            if record:
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record(stream)
-            ctx.check(Lb.ldga_chi_gamma_dev(ctx.handle, s, prec, frp, mP.U, mP.β, Γ.data_ptr(), χ0.data_ptr(),
+        if e: e[1].record(stream)
+        for s, Γ, χ, γ in ((1, Γm, o["χm"], o["γm"]), (-1, Γd, o["χd"], o["γd"])):
+            ctx.check(Lb.ldga_chi_gamma_dev(ctx.handle, s, PREC[prec], frp, mP.U, mP.β, Γ.data_ptr(), χ0.data_ptr(),
                                             χ.data_ptr(), γ.data_ptr(), None, sp))
-            if record:
-                e1.record(stream)
-                ev.append((e0, e1))
+        if e: e[2].record(stream)
         ctx.check(Lb.ldga_lambda0_dev(ctx.handle, frp, mP.β, Fm.data_ptr(), χ0.data_ptr(), λ0.data_ptr(), sp))
-        ctx.check(Lb.ldga_sigma_dev(ctx.handle, frp, mop, numax, 0.0, 0.0, 1, 0, inp.χloc_m_sum, χm.data_ptr(),
-                                    γm.data_ptr(), χd.data_ptr(), γd.data_ptr(), λ0.data_ptr(), Gr.data_ptr(),
-                                    Σ.data_ptr(), sp))
+        if e: e[3].record(stream)
+        ctx.check(Lb.ldga_sigma_dev(ctx.handle, frp, mop, numax, 0.0, 0.0, 1, 0, inp.χloc_m_sum, o["χm"].data_ptr(),
+                                    o["γm"].data_ptr(), o["χd"].data_ptr(), o["γd"].data_ptr(), λ0.data_ptr(),
+                                    Gr.data_ptr(), o["Σ"].data_ptr(), sp))
+        if e:
+            e[4].record(stream)
+            rec.append(e)
 
-    for _ in range(args.warmup):
-        step(False)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(True)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
+    def run(prec):
+        for _ in range(args.warmup):
+            step(prec, None)
+        torch.cuda.synchronize()
+        rec = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(prec, rec)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ph = np.mean([[a.elapsed_time(b) for a, b in zip(e[:-1], e[1:])] for e in rec], axis=0)
+        return wall * 1e3 / args.steps, dict(zip(["bubble", "bse_2ch", "lambda0", "eom"], ph.tolist()))
+
+    main_prec = args.precision
+    other = "fp64" if main_prec == "mixed" else "mixed"
+    ms_main, ph_main = run(main_prec)
+    ms_other, ph_other = run(other)
     pts = Nq * Nω
-    bse_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
-    bse_flops = Nq * Nω * (8.0 / 3.0) * Nν ** 3          # LU-equivalent, one channel per launch
+
+    def rel(a, b):
+        a, b = a.cpu().numpy(), b.cpu().numpy()
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    errs = {k: rel(out["mixed"][k], out["fp64"][k]) for k in ("Σ", "χm", "χd", "γm", "γd")}
+    bse_ms = ph_main["bse_2ch"] / 2                       # one channel per launch
+    bse_flops = Nq * Nω * (8.0 / 3.0) * Nν ** 3          # LU-equivalent, one channel
+    peak = FP64_PEAK_TFLOPS if main_prec == "fp64" else FP32_PEAK_TFLOPS
     achieved = bse_flops / (bse_ms * 1e-3) / 1e12
+    bub_bytes = 2 * Nk * Nfft * 16 + Nq * Nν * Nω * 16
+    eom_bytes = 3 * Nq * numax * Nω * 16 + 2 * Nq * Nω * 8 + Nk * (numax + Nω) * 16 + Nq * numax * 16
     result = {
         "metric": METRIC,
-        "value": pts / (ms_per_step * 1e-3),
+        "value": pts / (ms_main * 1e-3),
         "unit": "(q,ω)-points/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_main,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64-IR",
+        "dtype": "f64" if main_prec == "fp64" else "f32+f64-IR",
         "data": "synthetic (seeded atomic-limit Σ, noisy local vertices)",
         "config": {"workload": f"{cfg.D}D simple-cubic Ns={cfg.Ns} (Nq={Nq}) U={cfg.U} β={cfg.β} n={cfg.n} "
                                f"Nν={Nν} Nω={Nω} νmax={numax} shift={cfg.shift}, plain sums",
-                   "precision": args.precision, "points_per_step": pts},
-        "roofline": {"bound": "mfma", "kernel": "bse (one channel per launch)", "achieved": achieved,
-                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": MEASURED_TRAFFIC.get((args.config, args.precision)), "avg_launch_ms": bse_ms,
+                   "precision": main_prec, "points_per_step": pts},
+        "roofline": {"bound": "mfma", "kernel": f"bse {main_prec} (one channel per launch)", "achieved": achieved,
+                     "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": MEASURED_TRAFFIC.get((args.config, main_prec)), "avg_launch_ms": bse_ms,
                      "flops_per_launch": bse_flops},
+        "phases_ms": {k: round(v, 4) for k, v in ph_main.items()},
+        "hbm_GBps": {"bubble": bub_bytes / (ph_main["bubble"] * 1e-3) / 1e9,
+                     "eom": eom_bytes / (ph_main["eom"] * 1e-3) / 1e9},
+        f"{other}_gpu": {"value": pts / (ms_other * 1e-3), "ms_per_step": ms_other,
+                         "phases_ms": {k: round(v, 4) for k, v in ph_other.items()}},
+        "mixed_vs_fp64_gpu": (ms_other / ms_main) if main_prec == "mixed" else (ms_main / ms_other),
+        "mixed_rel_err_vs_fp64": errs,
     }
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(inp, numax, args.cpu_points)

@@ -222,10 +222,15 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
   // owner of panel kp+3 after the barrier of iteration kp+2.
   constexpr int NP = 4 * NT;
   static_assert(NT >= 2, "the pipeline needs two waves");
+  // Per-lane parts of the LDS addresses, re-laundered every iteration: the
+  // compiler then forms each address as base + immediate offset instead of
+  // keeping (and spilling) one hoisted register per array element.
+  int Lg = g, Ln = n, Ll = lane;
 
   // owner of panel kp1 (rows: element j1): stage its panel rows, updated by W[wb]
   // with the A operand rows in sh.C[wave] when pre, factor them, publish W_kp1.
   auto factor = [&](int kp1, int j1, bool pre, int wb) {
+    __builtin_amdgcn_s_setprio(3);
     with_j(j1, [&](auto JJ) {
       constexpr int J = decltype(JJ)::value;
       T xr[NT], xi[NT];
@@ -237,17 +242,17 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
       if (pre) {
 #pragma unroll
         for (int st = 0; st < 4; ++st) {
-          const c2 cr = sh.C[wave][MT::trow(g, J)][st];
+          const c2 cr = sh.C[wave][MT::trow(Lg, J)][st];
 #pragma unroll
           for (int t = 0; t < NT; ++t) {
-            const c2 bb = sh.W[wb][st][16 * t + n];
+            const c2 bb = sh.W[wb][st][16 * t + Ln];
             xr[t] = fma(cr.x, bb.x, fma(-cr.y, bb.y, xr[t]));
             xi[t] = fma(cr.x, bb.y, fma(cr.y, bb.x, xi[t]));
           }
         }
       }
 #pragma unroll
-      for (int t = 0; t < NT; ++t) sh.panel[g][16 * t + n] = mkc<T>(xr[t], xi[t]);
+      for (int t = 0; t < NT; ++t) sh.panel[Lg][16 * t + Ln] = mkc<T>(xr[t], xi[t]);
     });
     wave_sync_lds();
     T pr[NU][4], pim[NU][4];
@@ -257,7 +262,7 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         c2 v = mkc<T>(0, 0);
-        if (c < NC) v = sh.panel[r][c];
+        if (c < NC) v = sh.panel[r][Ll + 64 * u];
         pr[u][r] = v.x;
         pim[u][r] = v.y;
       }
@@ -343,7 +348,7 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
       const int c = lane + 64 * u;
       if (c < NC) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) sh.W[bo][r][c] = mkc<T>(pr[u][r] - (c == P[r] ? (T)1 : (T)0), pim[u][r]);
+        for (int r = 0; r < 4; ++r) sh.W[bo][r][Ll + 64 * u] = mkc<T>(pr[u][r] - (c == P[r] ? (T)1 : (T)0), pim[u][r]);
       }
     }
     if (lane < 4) {
@@ -353,6 +358,7 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
       sh.pcol_of_row[row] = p;
       sh.row_of_col[p] = row;
     }
+    __builtin_amdgcn_s_setprio(0);
   };
 
   c2 a_def = mkc<T>(0, 0);  // A operand of the owner's deferred update
@@ -360,6 +366,7 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
   for (int kp = 0; kp < NP; ++kp) {
     const int o = kp % NT, j = kp / NT, bw = kp % 3;
     const int kp1 = kp + 1, o1 = kp1 % NT;
+    __asm__ volatile("" : "+v"(Lg), "+v"(Ln), "+v"(Ll));
     __syncthreads();  // W_kp, P_kp published
     int Ps[4];
 #pragma unroll
@@ -373,7 +380,7 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
       const int bp = (kp + 2) % 3;
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        const c2 bb = sh.W[bp][g][16 * t + n];
+        const c2 bb = sh.W[bp][Lg][16 * t + Ln];
         Vr[t] = MT::mma(a_def.x, bb.x, Vr[t]);
         Vr[t] = MT::mma(-a_def.y, bb.y, Vr[t]);
         Vi[t] = MT::mma(a_def.x, bb.y, Vi[t]);
@@ -387,18 +394,18 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
       tile_select<NT>(Vr, Vi, Ps[st] >> 4, xr, xi);  // uniform tile index: scalar branches only
       if (n == (Ps[st] & 15)) {
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj) sh.C[wave][MT::trow(g, jj)][st] = mkc<T>(xr[jj], xi[jj]);
+        for (int jj = 0; jj < 4; ++jj) sh.C[wave][MT::trow(Lg, jj)][st] = mkc<T>(xr[jj], xi[jj]);
       }
     }
     wave_sync_lds();
-    const c2 a = sh.C[wave][n][g];  // A operand: tile row n, k = g
+    const c2 a = sh.C[wave][Ln][Lg];  // A operand: tile row n, k = g
     if (kp1 < NP && wave == o1) {
       factor(kp1, kp1 / NT, true, bw);
       a_def = a;
     } else {
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        const c2 bb = sh.W[bw][g][16 * t + n];  // B operand: k = g, column 16t + n
+        const c2 bb = sh.W[bw][Lg][16 * t + Ln];  // B operand: k = g, column 16t + n
         Vr[t] = MT::mma(a.x, bb.x, Vr[t]);
         Vr[t] = MT::mma(-a.y, bb.y, Vr[t]);
         Vi[t] = MT::mma(a.x, bb.y, Vi[t]);
@@ -411,7 +418,7 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
         constexpr int J = decltype(JJ)::value;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-          const c2 v = sh.W[bw][g][16 * t + n];
+          const c2 v = sh.W[bw][Lg][16 * t + Ln];
           Vr[t][J] = v.x + ((16 * t + n) == pg ? (T)1 : (T)0);
           Vi[t][J] = v.y;
         }
