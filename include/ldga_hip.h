@@ -69,8 +69,9 @@ typedef struct {
   int64_t n_matrices;    /* matrices factorised                              */
   int64_t n_zero_pivot;  /* matrices with an exactly zero pivot              */
   int32_t ir_steps_max;  /* mixed mode: largest refinement step count used   */
-  int32_t n_fallback;    /* mixed mode: matrices re-solved in FP64           */
-  double max_rel_resid;  /* mixed mode: largest final |r|/|b| (FP64)         */
+  int32_t n_fallback;    /* mixed mode: matrices whose refinement ended above 2^-30 */
+  double max_rel_resid;  /* mixed mode: largest final correction |dy|/|y|    */
+  double ir_steps_mean;  /* mixed mode: mean refinement step count           */
 } ldga_bse_report;
 
 /* ---- context --------------------------------------------------------- */

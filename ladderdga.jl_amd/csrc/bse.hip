@@ -286,7 +286,8 @@ int launch_bse(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double 
   if (s != 1 && s != -1) return ldga_fail(ctx, LDGA_ERR_ARG, "channel sign s must be +1 (m) or -1 (d)");
   const int N = 2 * fr->n_inu, Nw = 2 * fr->n_iw + 1, Nq = ctx->Nq, Nc = N + 2 * fr->n_inu_shell;
   // counters: [0] zero pivots, [1] refinement steps max, [2] unconverged refinements,
-  // [3] largest final refinement ratio (bits), [4] pipeline errors of the blocked kernel
+  // [3] largest final refinement ratio (bits), [4] pipeline errors of the blocked kernel,
+  // [5] refinement steps summed over the matrices
   unsigned long long *cnt = (unsigned long long *)ldga_buf(ctx, "bse_counters", 8 * 8);
   if (!cnt) return LDGA_ERR_HIP;
   LDGA_HIP_TRY(ctx, hipMemsetAsync(cnt, 0, 8 * 8, st));
@@ -311,7 +312,7 @@ int launch_bse(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double 
   }
   if (rc) return rc;
   if (rep) {
-    unsigned long long h[5];
+    unsigned long long h[6];
     LDGA_HIP_TRY(ctx, hipMemcpyAsync(h, cnt, sizeof h, hipMemcpyDeviceToHost, st));
     LDGA_HIP_TRY(ctx, hipStreamSynchronize(st));
     rep->n_matrices = (int64_t)Nq * Nw;
@@ -321,6 +322,7 @@ int launch_bse(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double 
     double r;
     memcpy(&r, &h[3], 8);
     rep->max_rel_resid = r;
+    rep->ir_steps_mean = (double)h[5] / (double)((int64_t)Nq * Nw);
     if (h[4]) return ldga_fail(ctx, LDGA_ERR_STATE, "%llu BSE workgroups hit the staging wait bound", h[4]);
     if (h[0]) return ldga_fail(ctx, LDGA_ERR_ZERO_PIVOT, "%llu BSE matrices had an exactly zero pivot", h[0]);
   }

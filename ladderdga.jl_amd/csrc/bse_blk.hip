@@ -456,8 +456,13 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
         const int i = ib + g;
         double2 acc = cz(0.0, 0.0);
         if (i < N) {
+          // all NT loads of the row in flight at once (L2-resident Gamma)
           const double2 *col = G + (size_t)N * i;
-          for (int jx = n; jx < N; jx += 16) acc = cadd(acc, cmul(col[jx], sh.y[jx]));
+          double2 gv[NT];
+#pragma unroll
+          for (int k = 0; k < NT; ++k) gv[k] = (n + 16 * k < N) ? col[n + 16 * k] : cz(0.0, 0.0);
+#pragma unroll
+          for (int k = 0; k < NT; ++k) acc = cadd(acc, cmul(gv[k], sh.y[n + 16 * k]));
         }
         acc.x = row_sum16(acc.x);
         acc.y = row_sum16(acc.y);
@@ -512,9 +517,13 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
       }
       const double ratio = dm / ym;
       __syncthreads();  // red[] is rewritten by the next step
-      // converged, or stagnating at the FP64 residual's own rounding level
-      // (kappa(A) * 2^-53: the correction no longer halves)
-      const bool stop = !(ratio > 0x1p-50) || (it > 0 && !(ratio < 0.5 * last_ratio));
+      // Stop when converged: the correction is at FP64 rounding, or the next
+      // one, extrapolated from the observed contraction ratio / last_ratio, is;
+      // or when stagnating at the FP64 residual's own rounding level
+      // (kappa(A) * 2^-53: the correction no longer halves).
+      const bool converged = !(ratio > 0x1p-50) || (it > 0 && ratio * ratio <= 0x1p-52 * last_ratio);
+      const bool stagnant = it > 0 && !(ratio < 0.5 * last_ratio);
+      const bool stop = converged || stagnant;
       last_ratio = ratio;
       if (stop) {
         ++it;
@@ -523,6 +532,7 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
     }
     if (tid == 0) {
       atomicMax(&counters[1], (unsigned long long)it);
+      atomicAdd(&counters[5], (unsigned long long)it);
       if (!(last_ratio <= 0x1p-30)) atomicAdd(&counters[2], 1ull);
       unsigned long long bits;
       const double lr = last_ratio == last_ratio ? last_ratio : 1.0;

@@ -78,7 +78,7 @@ def calc_χγ(type_: str, Γr, χ0: χ0T, kG: KGrid, mP: ModelParameters, sP: Si
                       f"(largest final correction {rep.max_rel_resid:.3g} of |y|)")
     if report is not None:
         report.update(n_matrices=rep.n_matrices, n_zero_pivot=rep.n_zero_pivot, ir_steps_max=rep.ir_steps_max,
-                      n_fallback=rep.n_fallback, max_rel_resid=rep.max_rel_resid)
+                      n_fallback=rep.n_fallback, max_rel_resid=rep.max_rel_resid, ir_steps_mean=rep.ir_steps_mean)
     return χT(χ, mP.β, tail_c=[0.0, 0.0, mP.Ekin_DMFT]), γT(γ)
 
 
