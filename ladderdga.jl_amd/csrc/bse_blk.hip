@@ -146,7 +146,7 @@ struct BlkShared {
   typedef typename mfma_traits<T>::c2 c2;
   static constexpr int NC = 16 * NT;
   c2 W[3][4][NC];      // published panel update (triple-buffered)
-  c2 panel[4][NC];     // owner's staging of the panel rows
+  c2 panel[2][4][NC];  // staged panel rows (double-buffered, per owner)
   c2 C[NT][16][4];     // per wave: own rows at the four pivot columns
   int P[3][4];         // pivot columns of the panel
   int pcol_of_row[NC]; // pivot column chosen for each row
@@ -214,12 +214,17 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
 
   // ---- panels: a two-stage software pipeline -----------------------------------
   // Panel kp = element kp / NT of the lane groups of wave kp % NT.  Its owner
-  // factors it one iteration early: during iteration kp-1 it applies W_{kp-1}
-  // to its four panel rows only (VALU), factors panel kp and publishes W_kp, and
-  // it applies W_{kp-1} to its whole tile (MFMA) at the start of iteration kp.
-  // The other waves' MFMA updates thus overlap the serial pivot chain.  W and P
-  // are triple-buffered: W_kp is read up to iteration kp+1 and rewritten by the
-  // owner of panel kp+3 after the barrier of iteration kp+2.
+  // stages its raw panel rows in LDS at the end of iteration kp-2 (after the
+  // MFMA update by W_{kp-2}) and factors the panel one iteration early: in
+  // iteration kp-1 it applies W_{kp-1} to the staged rows one column per lane
+  // (the pivot-column values come from the same copy), factors them and
+  // publishes W_kp and goes straight to the barrier; the MFMA update of its
+  // whole tile by W_{kp-1} (A operand gathered then) follows in iteration kp.  The serial chain per iteration is thus:
+  // barrier, rank-4 update of four staged rows, four pivot steps, publish; the
+  // other waves' MFMA updates overlap it.  W and P are triple-buffered (W_kp is
+  // read up to iteration kp+1 and rewritten by the owner of panel kp+3 after
+  // the barrier of iteration kp+2); the staging buffer is double-buffered and
+  // written and read by the same wave.
   constexpr int NP = 4 * NT;
   static_assert(NT >= 2, "the pipeline needs two waves");
   // Per-lane parts of the LDS addresses, re-laundered every iteration: the
@@ -227,34 +232,50 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
   // keeping (and spilling) one hoisted register per array element.
   int Lg = g, Ln = n, Ll = lane;
 
-  // owner of panel kp1 (rows: element j1): stage its panel rows, updated by W[wb]
-  // with the A operand rows in sh.C[wave] when pre, factor them, publish W_kp1.
-  auto factor = [&](int kp1, int j1, bool pre, int wb) {
-    __builtin_amdgcn_s_setprio(3);
-    with_j(j1, [&](auto JJ) {
+  // raw rows of panel m (this wave's element m / NT) -> staging buffer m & 1
+  auto stage_rows = [&](int m) {
+    const int sb = m & 1;
+    with_j(m / NT, [&](auto JJ) {
       constexpr int J = decltype(JJ)::value;
-      T xr[NT], xi[NT];
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        xr[t] = Vr[t][J];
-        xi[t] = Vi[t][J];
-      }
-      if (pre) {
+      for (int t = 0; t < NT; ++t) sh.panel[sb][Lg][16 * t + Ln] = mkc<T>(Vr[t][J], Vi[t][J]);
+    });
+  };
+
+  // owner of panel kp1: load its staged rows one column per lane, apply panel
+  // kp1-1 (pivots q0..q3, W[wb]) when pre, factor them and publish W_kp1.
+  auto factor = [&](int kp1, int j1, bool pre, int wb, int q0, int q1, int q2, int q3) {
+    __builtin_amdgcn_s_setprio(3);
+    const int sb = kp1 & 1;
+    wave_sync_lds();
+    if (pre) {
+      // staged rows += rows[:, Q] W, in place in LDS one row at a time (the
+      // staged values at the pivot columns are the rows' A operand; every lane
+      // reads them before any lane rewrites the row)
 #pragma unroll
-        for (int st = 0; st < 4; ++st) {
-          const c2 cr = sh.C[wave][MT::trow(Lg, J)][st];
+      for (int r = 0; r < 4; ++r) {
+        const c2 c0 = sh.panel[sb][r][q0], c1 = sh.panel[sb][r][q1];
+        const c2 c2_ = sh.panel[sb][r][q2], c3 = sh.panel[sb][r][q3];
 #pragma unroll
-          for (int t = 0; t < NT; ++t) {
-            const c2 bb = sh.W[wb][st][16 * t + Ln];
-            xr[t] = fma(cr.x, bb.x, fma(-cr.y, bb.y, xr[t]));
-            xi[t] = fma(cr.x, bb.y, fma(cr.y, bb.x, xi[t]));
+        for (int u = 0; u < NU; ++u) {
+          if (lane + 64 * u < NC) {
+            const int cl = Ll + 64 * u;
+            c2 v = sh.panel[sb][r][cl];
+            const c2 w0 = sh.W[wb][0][cl], w1 = sh.W[wb][1][cl], w2 = sh.W[wb][2][cl], w3 = sh.W[wb][3][cl];
+            v.x = fma(c0.x, w0.x, fma(-c0.y, w0.y, v.x));
+            v.y = fma(c0.x, w0.y, fma(c0.y, w0.x, v.y));
+            v.x = fma(c1.x, w1.x, fma(-c1.y, w1.y, v.x));
+            v.y = fma(c1.x, w1.y, fma(c1.y, w1.x, v.y));
+            v.x = fma(c2_.x, w2.x, fma(-c2_.y, w2.y, v.x));
+            v.y = fma(c2_.x, w2.y, fma(c2_.y, w2.x, v.y));
+            v.x = fma(c3.x, w3.x, fma(-c3.y, w3.y, v.x));
+            v.y = fma(c3.x, w3.y, fma(c3.y, w3.x, v.y));
+            sh.panel[sb][r][cl] = v;
           }
         }
       }
-#pragma unroll
-      for (int t = 0; t < NT; ++t) sh.panel[Lg][16 * t + Ln] = mkc<T>(xr[t], xi[t]);
-    });
-    wave_sync_lds();
+      wave_sync_lds();
+    }
     T pr[NU][4], pim[NU][4];
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
@@ -262,7 +283,7 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         c2 v = mkc<T>(0, 0);
-        if (c < NC) v = sh.panel[r][Ll + 64 * u];
+        if (c < NC) v = sh.panel[sb][r][Ll + 64 * u];
         pr[u][r] = v.x;
         pim[u][r] = v.y;
       }
@@ -361,8 +382,35 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
     __builtin_amdgcn_s_setprio(0);
   };
 
-  c2 a_def = mkc<T>(0, 0);  // A operand of the owner's deferred update
-  if (wave == 0) factor(0, 0, false, 0);
+  // this wave's tile += tile[:, Q] W[wb]: gather the A operand (own rows at
+  // the pivot columns q0..q3) through LDS, then four K=4 MFMAs per tile
+  auto update_tile = [&](int q0, int q1, int q2, int q3, int wb) {
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const int q = st == 0 ? q0 : st == 1 ? q1 : st == 2 ? q2 : q3;
+      v4 xr, xi;
+      tile_select<NT>(Vr, Vi, q >> 4, xr, xi);  // uniform tile index: scalar branches only
+      if (n == (q & 15)) {
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) sh.C[wave][MT::trow(Lg, jj)][st] = mkc<T>(xr[jj], xi[jj]);
+      }
+    }
+    wave_sync_lds();
+    const c2 a = sh.C[wave][Ln][Lg];  // A operand: tile row n, k = g
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const c2 bb = sh.W[wb][Lg][16 * t + Ln];  // B operand: k = g, column 16t + n
+      Vr[t] = MT::mma(a.x, bb.x, Vr[t]);
+      Vr[t] = MT::mma(-a.y, bb.y, Vr[t]);
+      Vi[t] = MT::mma(a.x, bb.y, Vi[t]);
+      Vi[t] = MT::mma(a.y, bb.x, Vi[t]);
+    }
+    __builtin_amdgcn_wave_barrier();
+  };
+
+  if (wave == 0) stage_rows(0);
+  if (wave == 1) stage_rows(1);
+  if (wave == 0) factor(0, 0, false, 0, 0, 0, 0, 0);
   for (int kp = 0; kp < NP; ++kp) {
     const int o = kp % NT, j = kp / NT, bw = kp % 3;
     const int kp1 = kp + 1, o1 = kp1 % NT;
@@ -376,53 +424,31 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
       for (int st = 0; st < 4; ++st)
         if ((Ps[st] & 63) == lane) pmask |= 1u << (Ps[st] >> 6);
     }
-    if (wave == o && kp > 0) {  // the update by W_{kp-1} deferred while this wave factored panel kp
-      const int bp = (kp + 2) % 3;
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const c2 bb = sh.W[bp][Lg][16 * t + Ln];
-        Vr[t] = MT::mma(a_def.x, bb.x, Vr[t]);
-        Vr[t] = MT::mma(-a_def.y, bb.y, Vr[t]);
-        Vi[t] = MT::mma(a_def.x, bb.y, Vi[t]);
-        Vi[t] = MT::mma(a_def.y, bb.x, Vi[t]);
-      }
-    }
-    // own rows at the four pivot columns of panel kp (the MFMA A operand)
-#pragma unroll
-    for (int st = 0; st < 4; ++st) {
-      v4 xr, xi;
-      tile_select<NT>(Vr, Vi, Ps[st] >> 4, xr, xi);  // uniform tile index: scalar branches only
-      if (n == (Ps[st] & 15)) {
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) sh.C[wave][MT::trow(Lg, jj)][st] = mkc<T>(xr[jj], xi[jj]);
-      }
-    }
-    wave_sync_lds();
-    const c2 a = sh.C[wave][Ln][Lg];  // A operand: tile row n, k = g
     if (kp1 < NP && wave == o1) {
-      factor(kp1, kp1 / NT, true, bw);
-      a_def = a;
+      // the critical path: straight from the barrier to the next barrier; this
+      // wave's own MFMA update by W_kp is applied in the next iteration
+      factor(kp1, kp1 / NT, true, bw, Ps[0], Ps[1], Ps[2], Ps[3]);
     } else {
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const c2 bb = sh.W[bw][Lg][16 * t + Ln];  // B operand: k = g, column 16t + n
-        Vr[t] = MT::mma(a.x, bb.x, Vr[t]);
-        Vr[t] = MT::mma(-a.y, bb.y, Vr[t]);
-        Vi[t] = MT::mma(a.x, bb.y, Vi[t]);
-        Vi[t] = MT::mma(a.y, bb.x, Vi[t]);
+      if (wave == o && kp > 0) {  // the update by W_{kp-1} deferred while this wave factored panel kp
+        const int bp = (kp + 2) % 3;
+        update_tile(__builtin_amdgcn_readfirstlane(sh.P[bp][0]), __builtin_amdgcn_readfirstlane(sh.P[bp][1]),
+                    __builtin_amdgcn_readfirstlane(sh.P[bp][2]), __builtin_amdgcn_readfirstlane(sh.P[bp][3]), bp);
       }
-    }
-    if (wave == o) {  // panel rows: their final values
-      const int pg = Ps[0] * (g == 0) + Ps[1] * (g == 1) + Ps[2] * (g == 2) + Ps[3] * (g == 3);
-      with_j(j, [&](auto JJ) {
-        constexpr int J = decltype(JJ)::value;
+      update_tile(Ps[0], Ps[1], Ps[2], Ps[3], bw);
+      if (wave == o) {  // panel rows: their final values
+        const int pg = Ps[0] * (g == 0) + Ps[1] * (g == 1) + Ps[2] * (g == 2) + Ps[3] * (g == 3);
+        with_j(j, [&](auto JJ) {
+          constexpr int J = decltype(JJ)::value;
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          const c2 v = sh.W[bw][Lg][16 * t + Ln];
-          Vr[t][J] = v.x + ((16 * t + n) == pg ? (T)1 : (T)0);
-          Vi[t][J] = v.y;
-        }
-      });
+          for (int t = 0; t < NT; ++t) {
+            const c2 v = sh.W[bw][Lg][16 * t + Ln];
+            Vr[t][J] = v.x + ((16 * t + n) == pg ? (T)1 : (T)0);
+            Vi[t][J] = v.y;
+          }
+        });
+      }
+      // the owner of panel kp+2 stages its rows (updated through panel kp)
+      if (kp + 2 < NP && wave == (kp + 2) % NT) stage_rows(kp + 2);
     }
     __builtin_amdgcn_wave_barrier();
   }
