@@ -18,11 +18,11 @@
 // column 16t+n of tile t and the four rows trow(g, j), j = 0..3, in element j.
 //
 // Rows are processed in panels of four: panel (j, w) = element j of every
-// lane group of wave w.  The owner wave copies the panel to LDS, reloads it
-// one column per lane (columns lane + 64u), runs the four pivot steps there
-// (64-lane DPP argmax, readlane broadcasts, no workgroup barrier), and
-// publishes W = (panel after) - E_P.  After one barrier every wave applies
-// the panel to its own rows as a rank-4 block update on MFMA,
+// lane group of wave w.  The owner wave stages its rows in LDS, loads the
+// panel rows one column per lane (columns lane + 64u), runs the four pivot
+// steps there (64-lane DPP argmax, readlane broadcasts, no workgroup
+// barrier), and publishes W = (panel after) - E_P.  After one barrier every
+// wave applies the panel to its own rows as a rank-4 block update on MFMA,
 //     M[S,:] += M[S,P] W     (S = rows outside the panel, M[S,P] before),
 // the complex product split into four real K=4 products (4M scheme).  The
 // panel ownership rotates over the waves.
@@ -105,21 +105,6 @@ __device__ __forceinline__ void with_j(int j, F &&f) {
   }
 }
 
-// Tile t (wave-uniform) of the accumulators.  The empty asm in each branch
-// keeps the compiler from folding the branches into a dynamically indexed
-// array, which would move the accumulators to scratch memory.
-template <int NT, typename V>
-__device__ __forceinline__ void tile_select(const V (&Vr)[NT], const V (&Vi)[NT], int t, V &xr, V &xi) {
-  t = __builtin_amdgcn_readfirstlane(t);
-#pragma unroll
-  for (int k = 0; k < NT; ++k)
-    if (t == k) {
-      xr = Vr[k];
-      xi = Vi[k];
-      __asm__ volatile("" : "+v"(xr), "+v"(xi));
-    }
-}
-
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double fast_rcp(double x) { return __builtin_amdgcn_rcp(x); }
 
@@ -141,13 +126,84 @@ __device__ __forceinline__ typename mfma_traits<T>::c2 mkc(T x, T y) {
   return r;
 }
 
+// One plane (real or imaginary) of a wave's accumulators: NT tiles of four
+// elements.  FP32 with NT <= 8 keeps them in one 32-register vector, so that a
+// tile picked by a wave-uniform runtime index is read by VGPR indexing (no
+// branches); otherwise they are NT 4-vectors and the tile is picked by scalar
+// branches.
+template <typename T, int NT, bool BANK = (sizeof(T) == 4 && NT <= 8)>
+struct AccPlane;
+
+template <typename T, int NT>
+struct AccPlane<T, NT, true> {
+  static constexpr bool bank = true;
+  typedef typename mfma_traits<T>::v4 v4;
+  typedef T bank_t __attribute__((ext_vector_type(NT <= 4 ? 16 : 32)));
+  bank_t b;
+  __device__ __forceinline__ void zero() { b = (bank_t)(T)0; }
+  // t compile-time (unrolled); j compile-time or wave-uniform
+  __device__ __forceinline__ T get(int t, int j) const { return b[4 * t + j]; }
+  __device__ __forceinline__ void set(int t, int j, T v) { b[4 * t + j] = v; }
+  __device__ __forceinline__ v4 tile(int t) const {
+    v4 r;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) r[j] = b[4 * t + j];
+    return r;
+  }
+  __device__ __forceinline__ void set_tile(int t, v4 v) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[4 * t + j] = v[j];
+  }
+  __device__ __forceinline__ bank_t &regs(int) { return b; }
+  // tile tq, a wave-uniform runtime index
+  __device__ __forceinline__ v4 select(int tq) const {
+    tq = __builtin_amdgcn_readfirstlane(tq);
+    v4 r;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) r[j] = b[4 * tq + j];
+    return r;
+  }
+};
+
+template <typename T, int NT>
+struct AccPlane<T, NT, false> {
+  static constexpr bool bank = false;
+  typedef typename mfma_traits<T>::v4 v4;
+  v4 v[NT];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) v[t] = (v4)(T)0;
+  }
+  // t and j compile-time (unrolled)
+  __device__ __forceinline__ T get(int t, int j) const { return v[t][j]; }
+  __device__ __forceinline__ void set(int t, int j, T x) { v[t][j] = x; }
+  __device__ __forceinline__ v4 tile(int t) const { return v[t]; }
+  __device__ __forceinline__ void set_tile(int t, v4 x) { v[t] = x; }
+  __device__ __forceinline__ v4 &regs(int t) { return v[t]; }
+  // tile tq, a wave-uniform runtime index.  The empty asm in each branch keeps
+  // the compiler from folding the branches into a dynamically indexed array,
+  // which would move the accumulators to scratch memory.
+  __device__ __forceinline__ v4 select(int tq) const {
+    tq = __builtin_amdgcn_readfirstlane(tq);
+    v4 r = v[0];
+#pragma unroll
+    for (int k = 1; k < NT; ++k)
+      if (tq == k) {
+        r = v[k];
+        __asm__ volatile("" : "+v"(r));
+      }
+    return r;
+  }
+};
+
 template <typename T, int NT>
 struct BlkShared {
   typedef typename mfma_traits<T>::c2 c2;
   static constexpr int NC = 16 * NT;
   c2 W[3][4][NC];      // published panel update (triple-buffered)
-  c2 panel[2][4][NC];  // staged panel rows (double-buffered, per owner)
-  c2 C[NT][16][4];     // per wave: own rows at the four pivot columns
+  c2 S[3][16][NC];     // a panel owner's 16 rows, staged (triple-buffered)
+  T Cr[NT][16][4];     // per wave: own rows at the four pivot columns (real)
+  T Ci[NT][16][4];     // (imaginary)
   int P[3][4];         // pivot columns of the panel
   int pcol_of_row[NC]; // pivot column chosen for each row
   int row_of_col[NC];  // row whose pivot column is c
@@ -190,21 +246,51 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
   __syncthreads();
 
   // ---- M = A^T into the accumulators: M[i][c] = Gamma[c, i, w] + delta_ic / chi0[i]
-  v4 Vr[NT], Vi[NT];
+  // (identity on the padding).  The loads carry no branches, so the loads of a
+  // group of LG tiles are in flight together; groups are issued one after the
+  // other to bound the registers they occupy.
+  AccPlane<T, NT> Vr, Vi;
+  Vr.zero();
+  Vi.zero();
+  constexpr int LG = sizeof(T) == 4 ? 2 : 4;
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int c = 16 * t + n;
+  for (int t0 = 0; t0 < NT; t0 += LG) {
+    // the lane indices are laundered per group so that no address of a later
+    // group is formed (and held in registers) early
+    int nl = n, gl = g;
+    __asm__ volatile("" : "+v"(nl), "+v"(gl));
+    double2 v[LG][4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int i = 16 * wave + MT::trow(g, j);
-      double2 v = cz(i == c ? 1.0 : 0.0, 0.0);
-      if (i < N && c < N) {
-        v = G[c + (size_t)N * i];
-        if (i == c) v = cadd(v, sh.dinv[i]);
+    for (int tt = 0; tt < LG; ++tt) {
+      const int c = 16 * (t0 + tt) + nl;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int i = 16 * wave + MT::trow(gl, j);
+        const bool ok = t0 + tt < NT && i < N && c < N;
+        v[tt][j] = G[ok ? c + N * i : 0];
+        if (!ok) v[tt][j] = cz(0.0, 0.0);
       }
-      Vr[t][j] = (T)v.x;
-      Vi[t][j] = (T)v.y;
     }
+#pragma unroll
+    for (int tt = 0; tt < LG; ++tt) {
+      const int t = t0 + tt;
+      if (t < NT) {
+        if (t == wave) {  // the diagonal tile (uniform branch)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int i = 16 * wave + MT::trow(g, j);
+            if (n == MT::trow(g, j)) v[tt][j] = cadd(v[tt][j], i < N ? sh.dinv[i] : cz(1.0, 0.0));
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          Vr.set(t, j, (T)v[tt][j].x);
+          Vi.set(t, j, (T)v[tt][j].y);
+        }
+        __asm__ volatile("" : "+v"(Vr.regs(t)), "+v"(Vi.regs(t)));  // converted here, not later
+      }
+    }
+    __asm__ volatile("" ::: "memory");
   }
   unsigned pmask = 0;  // pivoted columns lane + 64u, bit u (panel layout)
 #pragma unroll
@@ -214,17 +300,20 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
 
   // ---- panels: a two-stage software pipeline -----------------------------------
   // Panel kp = element kp / NT of the lane groups of wave kp % NT.  Its owner
-  // stages its raw panel rows in LDS at the end of iteration kp-2 (after the
+  // stages all 16 of its rows in LDS at the end of iteration kp-2 (after its
   // MFMA update by W_{kp-2}) and factors the panel one iteration early: in
-  // iteration kp-1 it applies W_{kp-1} to the staged rows one column per lane
-  // (the pivot-column values come from the same copy), factors them and
-  // publishes W_kp and goes straight to the barrier; the MFMA update of its
-  // whole tile by W_{kp-1} (A operand gathered then) follows in iteration kp.  The serial chain per iteration is thus:
-  // barrier, rank-4 update of four staged rows, four pivot steps, publish; the
-  // other waves' MFMA updates overlap it.  W and P are triple-buffered (W_kp is
-  // read up to iteration kp+1 and rewritten by the owner of panel kp+3 after
-  // the barrier of iteration kp+2); the staging buffer is double-buffered and
-  // written and read by the same wave.
+  // iteration kp-1 it loads the panel rows from the staged copy, applies
+  // W_{kp-1} to them in registers (the rows' values at the pivot columns by
+  // readlane), factors them, publishes W_kp and goes straight to the
+  // barrier.  Its own tile misses the update by W_{kp-1}; in iteration kp it
+  // applies W_{kp-1} and W_kp together, both A operands read from the staged
+  // copy (no gather from the accumulators).  The serial chain per iteration is
+  // thus: barrier, rank-4 update of four panel rows, four pivot steps,
+  // publish; the other waves' MFMA updates overlap it.  W and P are
+  // triple-buffered (W_kp is read up to iteration kp+1 and rewritten by the
+  // owner of panel kp+3 after the barrier of iteration kp+2); the staged rows
+  // too (read by their owner in iterations kp-1 and kp, rewritten at the end
+  // of iteration kp+1).
   constexpr int NP = 4 * NT;
   static_assert(NT >= 2, "the pipeline needs two waves");
   // Per-lane parts of the LDS addresses, re-laundered every iteration: the
@@ -232,50 +321,87 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
   // keeping (and spilling) one hoisted register per array element.
   int Lg = g, Ln = n, Ll = lane;
 
-  // raw rows of panel m (this wave's element m / NT) -> staging buffer m & 1
+  // all 16 rows of this wave (the owner of panel m) -> staging buffer m % 3
   auto stage_rows = [&](int m) {
-    const int sb = m & 1;
-    with_j(m / NT, [&](auto JJ) {
-      constexpr int J = decltype(JJ)::value;
+    const int sb = m % 3;
 #pragma unroll
-      for (int t = 0; t < NT; ++t) sh.panel[sb][Lg][16 * t + Ln] = mkc<T>(Vr[t][J], Vi[t][J]);
-    });
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj)
+        sh.S[sb][MT::trow(Lg, jj)][16 * t + Ln] = mkc<T>(Vr.get(t, jj), Vi.get(t, jj));
+  };
+
+  // this wave's rows at the pivot columns q0..q3 -> Cr/Ci[wave] (the A operand
+  // of its update; the lanes holding column q write it from tile q / 16)
+  auto gather_cols = [&](int q0, int q1, int q2, int q3) {
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const int q = st == 0 ? q0 : st == 1 ? q1 : st == 2 ? q2 : q3;
+      if (n == (q & 15)) {
+        const v4 xr = Vr.select(q >> 4), xi = Vi.select(q >> 4);
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          sh.Cr[wave][MT::trow(Lg, jj)][st] = xr[jj];
+          sh.Ci[wave][MT::trow(Lg, jj)][st] = xi[jj];
+        }
+      }
+    }
+  };
+  // tile t += A W[wb]: four K=4 MFMAs (a: tile row n, k = g)
+  auto mma_tile = [&](int t, c2 a, int wb) {
+    const c2 bb = sh.W[wb][Lg][16 * t + Ln];  // B operand: k = g, column 16t + n
+    v4 cr = Vr.tile(t), ci = Vi.tile(t);
+    cr = MT::mma(a.x, bb.x, cr);
+    cr = MT::mma(-a.y, bb.y, cr);
+    ci = MT::mma(a.x, bb.y, ci);
+    ci = MT::mma(a.y, bb.x, ci);
+    Vr.set_tile(t, cr);
+    Vi.set_tile(t, ci);
+  };
+  // the owner of panel kp, after factoring it: its tile += A1 W[w1] + A2 W[w2]
+  // for panels kp-1 (pivots p, deferred) and kp (pivots q), with both A
+  // operands from its staged rows S[sb] (taken after panel kp-2):
+  // A1 = S[:, P], A2 = S[:, Q] + A1 W[w1][:, Q].  (The panel's own rows of the
+  // tile are overwritten with their final values afterwards.)
+  auto update_tile2 = [&](int sb, int p0, int p1, int p2, int p3, int w1, int q0, int q1, int q2, int q3, int w2) {
+    const int pg = Lg == 0 ? p0 : Lg == 1 ? p1 : Lg == 2 ? p2 : p3;
+    const int qg = Lg == 0 ? q0 : Lg == 1 ? q1 : Lg == 2 ? q2 : q3;
+    const c2 a1 = sh.S[sb][Ln][pg];
+    c2 a2 = sh.S[sb][Ln][qg];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const c2 c = sh.S[sb][Ln][r == 0 ? p0 : r == 1 ? p1 : r == 2 ? p2 : p3];
+      const c2 wv = sh.W[w1][r][qg];
+      a2.x = fma(c.x, wv.x, fma(-c.y, wv.y, a2.x));
+      a2.y = fma(c.x, wv.y, fma(c.y, wv.x, a2.y));
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) mma_tile(t, a1, w1);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) mma_tile(t, a2, w2);
+    __builtin_amdgcn_wave_barrier();
+  };
+  // this wave's tile += tile[:, Q] W[wb]
+  auto update_tile = [&](int q0, int q1, int q2, int q3, int wb) {
+    gather_cols(q0, q1, q2, q3);
+    wave_sync_lds();
+    const c2 a = mkc<T>(sh.Cr[wave][Ln][Lg], sh.Ci[wave][Ln][Lg]);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) mma_tile(t, a, wb);
+    __builtin_amdgcn_wave_barrier();
   };
 
   // owner of panel kp1: load its staged rows one column per lane, apply panel
   // kp1-1 (pivots q0..q3, W[wb]) when pre, factor them and publish W_kp1.
-  auto factor = [&](int kp1, int j1, bool pre, int wb, int q0, int q1, int q2, int q3) {
+  auto factor = [&](auto PRE, int kp1, int j1, int wb, int q0, int q1, int q2, int q3) {
+    constexpr bool pre = decltype(PRE)::value;
     __builtin_amdgcn_s_setprio(3);
-    const int sb = kp1 & 1;
+    const int sb = kp1 % 3;
     wave_sync_lds();
-    if (pre) {
-      // staged rows += rows[:, Q] W, in place in LDS one row at a time (the
-      // staged values at the pivot columns are the rows' A operand; every lane
-      // reads them before any lane rewrites the row)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const c2 c0 = sh.panel[sb][r][q0], c1 = sh.panel[sb][r][q1];
-        const c2 c2_ = sh.panel[sb][r][q2], c3 = sh.panel[sb][r][q3];
-#pragma unroll
-        for (int u = 0; u < NU; ++u) {
-          if (lane + 64 * u < NC) {
-            const int cl = Ll + 64 * u;
-            c2 v = sh.panel[sb][r][cl];
-            const c2 w0 = sh.W[wb][0][cl], w1 = sh.W[wb][1][cl], w2 = sh.W[wb][2][cl], w3 = sh.W[wb][3][cl];
-            v.x = fma(c0.x, w0.x, fma(-c0.y, w0.y, v.x));
-            v.y = fma(c0.x, w0.y, fma(c0.y, w0.x, v.y));
-            v.x = fma(c1.x, w1.x, fma(-c1.y, w1.y, v.x));
-            v.y = fma(c1.x, w1.y, fma(c1.y, w1.x, v.y));
-            v.x = fma(c2_.x, w2.x, fma(-c2_.y, w2.y, v.x));
-            v.y = fma(c2_.x, w2.y, fma(c2_.y, w2.x, v.y));
-            v.x = fma(c3.x, w3.x, fma(-c3.y, w3.y, v.x));
-            v.y = fma(c3.x, w3.y, fma(c3.y, w3.x, v.y));
-            sh.panel[sb][r][cl] = v;
-          }
-        }
-      }
-      wave_sync_lds();
-    }
+    // the panel's rows in the staged copy: element j1 of the four lane groups
+    const int pr0 = MT::trow(0, j1), pr1 = MT::trow(1, j1), pr2 = MT::trow(2, j1), pr3 = MT::trow(3, j1);
+    auto prow = [&](int r) { return r == 0 ? pr0 : r == 1 ? pr1 : r == 2 ? pr2 : pr3; };
+    // the panel rows from the staged copy, one column per lane
     T pr[NU][4], pim[NU][4];
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
@@ -283,9 +409,47 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         c2 v = mkc<T>(0, 0);
-        if (c < NC) v = sh.panel[sb][r][Ll + 64 * u];
+        if (c < NC) v = sh.S[sb][prow(r)][Ll + 64 * u];
         pr[u][r] = v.x;
         pim[u][r] = v.y;
+      }
+    }
+    if constexpr (pre) {
+      // panel rows += rows[:, Q] W[wb]: W one column per lane; the rows' values
+      // at the pivot columns (taken before the row changes) by readlane from
+      // the lanes holding those columns
+      c2 wv[4][NU];
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        const int c = lane + 64 * u;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) wv[k][u] = c < NC ? sh.W[wb][k][Ll + 64 * u] : mkc<T>(0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        T cx[4], cy[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int qk = k == 0 ? q0 : k == 1 ? q1 : k == 2 ? q2 : q3;
+          const int uk = qk >> 6, lk = qk & 63;
+          T xr = pr[0][r], xi = pim[0][r];
+#pragma unroll
+          for (int u = 1; u < NU; ++u)
+            if (uk == u) {
+              xr = pr[u][r];
+              xi = pim[u][r];
+            }
+          cx[k] = rdlane(xr, lk);
+          cy[k] = rdlane(xi, lk);
+        }
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            pr[u][r] = fma(cx[k], wv[k][u].x, fma(-cy[k], wv[k][u].y, pr[u][r]));
+            pim[u][r] = fma(cx[k], wv[k][u].y, fma(cy[k], wv[k][u].x, pim[u][r]));
+          }
+        }
       }
     }
     int P[4];
@@ -349,16 +513,16 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
         pim[u][st] = -rsy;
       }
       // pivot column: M[i][p] = M[i][p] / piv, M[r][p] = 1 / piv
+      // (as selects: a taken branch costs more than the selects)
 #pragma unroll
-      for (int u = 0; u < NU; ++u) {
-        if (u == up) {  // uniform
-          if (lane == lp) {
+      for (int r = 0; r < 4; ++r) {
+        const T nx = r == st ? ivx : cpx[r] * ivx - cpy[r] * ivy;
+        const T ny = r == st ? ivy : cpx[r] * ivy + cpy[r] * ivx;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              pr[u][r] = r == st ? ivx : cpx[r] * ivx - cpy[r] * ivy;
-              pim[u][r] = r == st ? ivy : cpx[r] * ivy + cpy[r] * ivx;
-            }
-          }
+        for (int u = 0; u < NU; ++u) {
+          const bool hit = u == up && lane == lp;
+          pr[u][r] = hit ? nx : pr[u][r];
+          pim[u][r] = hit ? ny : pim[u][r];
         }
       }
     }
@@ -382,35 +546,9 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
     __builtin_amdgcn_s_setprio(0);
   };
 
-  // this wave's tile += tile[:, Q] W[wb]: gather the A operand (own rows at
-  // the pivot columns q0..q3) through LDS, then four K=4 MFMAs per tile
-  auto update_tile = [&](int q0, int q1, int q2, int q3, int wb) {
-#pragma unroll
-    for (int st = 0; st < 4; ++st) {
-      const int q = st == 0 ? q0 : st == 1 ? q1 : st == 2 ? q2 : q3;
-      v4 xr, xi;
-      tile_select<NT>(Vr, Vi, q >> 4, xr, xi);  // uniform tile index: scalar branches only
-      if (n == (q & 15)) {
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) sh.C[wave][MT::trow(Lg, jj)][st] = mkc<T>(xr[jj], xi[jj]);
-      }
-    }
-    wave_sync_lds();
-    const c2 a = sh.C[wave][Ln][Lg];  // A operand: tile row n, k = g
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const c2 bb = sh.W[wb][Lg][16 * t + Ln];  // B operand: k = g, column 16t + n
-      Vr[t] = MT::mma(a.x, bb.x, Vr[t]);
-      Vr[t] = MT::mma(-a.y, bb.y, Vr[t]);
-      Vi[t] = MT::mma(a.x, bb.y, Vi[t]);
-      Vi[t] = MT::mma(a.y, bb.x, Vi[t]);
-    }
-    __builtin_amdgcn_wave_barrier();
-  };
-
   if (wave == 0) stage_rows(0);
   if (wave == 1) stage_rows(1);
-  if (wave == 0) factor(0, 0, false, 0, 0, 0, 0, 0);
+  if (wave == 0) factor(std::false_type{}, 0, 0, 0, 0, 0, 0, 0);
   for (int kp = 0; kp < NP; ++kp) {
     const int o = kp % NT, j = kp / NT, bw = kp % 3;
     const int kp1 = kp + 1, o1 = kp1 % NT;
@@ -427,25 +565,28 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
     if (kp1 < NP && wave == o1) {
       // the critical path: straight from the barrier to the next barrier; this
       // wave's own MFMA update by W_kp is applied in the next iteration
-      factor(kp1, kp1 / NT, true, bw, Ps[0], Ps[1], Ps[2], Ps[3]);
+      factor(std::true_type{}, kp1, kp1 / NT, bw, Ps[0], Ps[1], Ps[2], Ps[3]);
     } else {
-      if (wave == o && kp > 0) {  // the update by W_{kp-1} deferred while this wave factored panel kp
+      if (wave == o && kp > 0) {  // with the update by W_{kp-1}, deferred while this wave factored panel kp
         const int bp = (kp + 2) % 3;
-        update_tile(__builtin_amdgcn_readfirstlane(sh.P[bp][0]), __builtin_amdgcn_readfirstlane(sh.P[bp][1]),
-                    __builtin_amdgcn_readfirstlane(sh.P[bp][2]), __builtin_amdgcn_readfirstlane(sh.P[bp][3]), bp);
+        update_tile2(kp % 3, __builtin_amdgcn_readfirstlane(sh.P[bp][0]), __builtin_amdgcn_readfirstlane(sh.P[bp][1]),
+                     __builtin_amdgcn_readfirstlane(sh.P[bp][2]), __builtin_amdgcn_readfirstlane(sh.P[bp][3]), bp,
+                     Ps[0], Ps[1], Ps[2], Ps[3], bw);
+      } else {
+        update_tile(Ps[0], Ps[1], Ps[2], Ps[3], bw);
       }
-      update_tile(Ps[0], Ps[1], Ps[2], Ps[3], bw);
       if (wave == o) {  // panel rows: their final values
         const int pg = Ps[0] * (g == 0) + Ps[1] * (g == 1) + Ps[2] * (g == 2) + Ps[3] * (g == 3);
-        with_j(j, [&](auto JJ) {
-          constexpr int J = decltype(JJ)::value;
+        auto body = [&](auto J) {
 #pragma unroll
           for (int t = 0; t < NT; ++t) {
             const c2 v = sh.W[bw][Lg][16 * t + Ln];
-            Vr[t][J] = v.x + ((16 * t + n) == pg ? (T)1 : (T)0);
-            Vi[t][J] = v.y;
+            Vr.set(t, J, v.x + ((16 * t + n) == pg ? (T)1 : (T)0));
+            Vi.set(t, J, v.y);
           }
-        });
+        };
+        if constexpr (decltype(Vr)::bank) body(__builtin_amdgcn_readfirstlane(j));
+        else with_j(j, body);
       }
       // the owner of panel kp+2 stages its rows (updated through panel kp)
       if (kp + 2 < NP && wave == (kp + 2) % NT) stage_rows(kp + 2);
@@ -460,8 +601,8 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
     T sx = 0, sy = 0;
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      sx += Vr[t][jj];
-      sy += Vi[t][jj];
+      sx += Vr.get(t, jj);
+      sy += Vi.get(t, jj);
     }
     sx = row_sum16(sx);
     sy = row_sum16(sy);
@@ -509,8 +650,8 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
         for (int t = 0; t < NT; ++t) {
           const double2 rv = sh.res[sh.row_of_col[16 * t + n]];
           const T rx = (T)rv.x, ry = (T)rv.y;
-          sx += Vr[t][jj] * rx - Vi[t][jj] * ry;
-          sy += Vr[t][jj] * ry + Vi[t][jj] * rx;
+          sx += Vr.get(t, jj) * rx - Vi.get(t, jj) * ry;
+          sy += Vr.get(t, jj) * ry + Vi.get(t, jj) * rx;
         }
         sx = row_sum16(sx);
         sy = row_sum16(sy);
