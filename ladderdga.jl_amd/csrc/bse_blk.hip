@@ -18,11 +18,11 @@
 // column 16t+n of tile t and the four rows trow(g, j), j = 0..3, in element j.
 //
 // Rows are processed in panels of four: panel (j, w) = element j of every
-// lane group of wave w.  The owner wave stages its rows in LDS, loads the
-// panel rows one column per lane (columns lane + 64u), runs the four pivot
-// steps there (64-lane DPP argmax, readlane broadcasts, no workgroup
-// barrier), and publishes W = (panel after) - E_P.  After one barrier every
-// wave applies the panel to its own rows as a rank-4 block update on MFMA,
+// lane group of wave w.  The owner wave copies the panel to LDS, reloads it
+// one column per lane (columns lane + 64u), runs the four pivot steps there
+// (64-lane DPP argmax, readlane broadcasts, no workgroup barrier), and
+// publishes W = (panel after) - E_P.  After one barrier every wave applies
+// the panel to its own rows as a rank-4 block update on MFMA,
 //     M[S,:] += M[S,P] W     (S = rows outside the panel, M[S,P] before),
 // the complex product split into four real K=4 products (4M scheme).  The
 // panel ownership rotates over the waves.
@@ -206,11 +206,11 @@ struct BlkShared {
   T Ci[NT][16][4];     // (imaginary)
   int P[3][4];         // pivot columns of the panel
   int pcol_of_row[NC]; // pivot column chosen for each row
-  int row_of_col[NC];  // row whose pivot column is c
   double2 y[NC];       // solution (FP64)
   double2 dinv[NC];    // 1/chi0 on the diagonal (FP64)
-  double2 res[NC];     // refinement residual (FP64)
+  double2 res[NC];     // refinement residual (FP64), at the pivot column of its row
   double red[2][NT];
+  float redf[2][2][NT];
   int zero_pivot;
 };
 
@@ -300,20 +300,17 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
 
   // ---- panels: a two-stage software pipeline -----------------------------------
   // Panel kp = element kp / NT of the lane groups of wave kp % NT.  Its owner
-  // stages all 16 of its rows in LDS at the end of iteration kp-2 (after its
+  // stages its raw panel rows in LDS at the end of iteration kp-2 (after the
   // MFMA update by W_{kp-2}) and factors the panel one iteration early: in
-  // iteration kp-1 it loads the panel rows from the staged copy, applies
-  // W_{kp-1} to them in registers (the rows' values at the pivot columns by
-  // readlane), factors them, publishes W_kp and goes straight to the
-  // barrier.  Its own tile misses the update by W_{kp-1}; in iteration kp it
-  // applies W_{kp-1} and W_kp together, both A operands read from the staged
-  // copy (no gather from the accumulators).  The serial chain per iteration is
-  // thus: barrier, rank-4 update of four panel rows, four pivot steps,
-  // publish; the other waves' MFMA updates overlap it.  W and P are
-  // triple-buffered (W_kp is read up to iteration kp+1 and rewritten by the
-  // owner of panel kp+3 after the barrier of iteration kp+2); the staged rows
-  // too (read by their owner in iterations kp-1 and kp, rewritten at the end
-  // of iteration kp+1).
+  // iteration kp-1 it applies W_{kp-1} to the staged rows one column per lane
+  // (the pivot-column values come from the same copy), factors them and
+  // publishes W_kp and goes straight to the barrier; the MFMA update of its
+  // whole tile by W_{kp-1} (A operand gathered then) follows in iteration kp.  The serial chain per iteration is thus:
+  // barrier, rank-4 update of four staged rows, four pivot steps, publish; the
+  // other waves' MFMA updates overlap it.  W and P are triple-buffered (W_kp is
+  // read up to iteration kp+1 and rewritten by the owner of panel kp+3 after
+  // the barrier of iteration kp+2); the staging buffer is double-buffered and
+  // written and read by the same wave.
   constexpr int NP = 4 * NT;
   static_assert(NT >= 2, "the pipeline needs two waves");
   // Per-lane parts of the LDS addresses, re-laundered every iteration: the
@@ -526,7 +523,7 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
         }
       }
     }
-    // publish W = panel - E_P, the pivots and the row <-> column maps
+    // publish W = panel - E_P, the pivots and each row's pivot column
     const int bo = kp1 % 3;
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
@@ -541,7 +538,6 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
       const int row = 16 * wave + MT::trow(lane, j1);
       sh.P[bo][lane] = p;
       sh.pcol_of_row[row] = p;
-      sh.row_of_col[p] = row;
     }
     __builtin_amdgcn_s_setprio(0);
   };
@@ -615,10 +611,15 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
 
   if constexpr (REFINE) {
     // ---- FP64 iterative refinement with the FP32 inverse --------------------
+    // the pivot columns of this lane's four rows (fixed from here on)
+    int pcs[4];
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) pcs[jj] = sh.pcol_of_row[16 * wave + MT::trow(g, jj)];
     int it = 0;
     double last_ratio = 0.0;
     for (; it < max_ir; ++it) {
-      // r_i = 1 - sum_j Gamma[j, i] y_j - y_i / chi0_i; wave w: rows 4k + g for k = w, w + NT, ...
+      // r_i = 1 - sum_j Gamma[j, i] y_j - y_i / chi0_i, stored at column
+      // pcol_of_row[i] (where X consumes it); wave w: rows 4k + g for k = w, w + NT, ...
       for (int ib = 4 * wave; ib < N; ib += 4 * NT) {
         const int i = ib + g;
         double2 acc = cz(0.0, 0.0);
@@ -636,54 +637,61 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
         if (n == 0 && i < N) {
           const double2 yi = sh.y[i];
           const double2 d = cmul(yi, sh.dinv[i]);
-          sh.res[i] = cz(1.0 - acc.x - d.x, -acc.y - d.y);
+          sh.res[sh.pcol_of_row[i]] = cz(1.0 - acc.x - d.x, -acc.y - d.y);
         }
       }
       if (tid < NC - N) sh.res[N + tid] = cz(0.0, 0.0);
       __syncthreads();
-      // dy[p_r] = sum_c X[r][c] res[row_of(c)]  (FP32 products)
-      double dmax = 0.0, ymax = 0.0;
+      // dy[p_r] = sum_c X[r][c] r[row_of(c)], r[row_of(c)] stored at c  (FP32 products)
+      float dmax = 0.0f, ymax = 0.0f;
 #pragma unroll
       for (int jj = 0; jj < 4; ++jj) {
         T sx = 0, sy = 0;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-          const double2 rv = sh.res[sh.row_of_col[16 * t + n]];
+          const double2 rv = sh.res[16 * t + n];
           const T rx = (T)rv.x, ry = (T)rv.y;
           sx += Vr.get(t, jj) * rx - Vi.get(t, jj) * ry;
           sy += Vr.get(t, jj) * ry + Vi.get(t, jj) * rx;
         }
         sx = row_sum16(sx);
         sy = row_sum16(sy);
-        if (n == 0) {
-          const int row = 16 * wave + MT::trow(g, jj);
-          const int pc = sh.pcol_of_row[row];
-          if (pc < N) {
-            const double2 yo = sh.y[pc];
-            const double2 yn = cz(yo.x + (double)sx, yo.y + (double)sy);
-            sh.y[pc] = yn;
-            dmax = fmax(dmax, fabs((double)sx) + fabs((double)sy));
-            ymax = fmax(ymax, fabs(yn.x) + fabs(yn.y));
-          }
+        const int pc = pcs[jj];
+        if (n == 0 && pc < N) {
+          const double2 yo = sh.y[pc];
+          const double2 yn = cz(yo.x + (double)sx, yo.y + (double)sy);
+          sh.y[pc] = yn;
+          dmax = fmaxf(dmax, fabsf(sx) + fabsf(sy));
+          ymax = fmaxf(ymax, (float)(fabs(yn.x) + fabs(yn.y)));
         }
       }
-      for (int off = 32; off > 0; off >>= 1) {
-        dmax = fmax(dmax, __shfl_xor(dmax, off));
-        ymax = fmax(ymax, __shfl_xor(ymax, off));
+      // wave maxima by DPP (non-negative floats order like their bit patterns;
+      // a NaN stays the maximum), then across the waves through LDS
+      unsigned dk = __float_as_uint(dmax), yk = __float_as_uint(ymax);
+      dk = max(dk, (unsigned)dpp32<0xB1, 0xf>((int)dk, (int)dk));
+      yk = max(yk, (unsigned)dpp32<0xB1, 0xf>((int)yk, (int)yk));
+      dk = max(dk, (unsigned)dpp32<0x4E, 0xf>((int)dk, (int)dk));
+      yk = max(yk, (unsigned)dpp32<0x4E, 0xf>((int)yk, (int)yk));
+      dk = max(dk, (unsigned)dpp32<0x141, 0xf>((int)dk, (int)dk));
+      yk = max(yk, (unsigned)dpp32<0x141, 0xf>((int)yk, (int)yk));
+      dk = max(dk, (unsigned)dpp32<0x140, 0xf>((int)dk, (int)dk));
+      yk = max(yk, (unsigned)dpp32<0x140, 0xf>((int)yk, (int)yk));
+      dk = max(dk, (unsigned)dpp32<0x142, 0xa>((int)dk, (int)dk));
+      yk = max(yk, (unsigned)dpp32<0x142, 0xa>((int)yk, (int)yk));
+      dk = max(dk, (unsigned)dpp32<0x143, 0xc>((int)dk, (int)dk));
+      yk = max(yk, (unsigned)dpp32<0x143, 0xc>((int)yk, (int)yk));
+      if (lane == 63) {
+        sh.redf[it & 1][0][wave] = __uint_as_float(dk);
+        sh.redf[it & 1][1][wave] = __uint_as_float(yk);
       }
-      if (lane == 0) {
-        sh.red[0][wave] = dmax;
-        sh.red[1][wave] = ymax;
-      }
-      __syncthreads();
-      double dm = 0.0, ym = 0.0;
+      __syncthreads();  // redf[it & 1] is rewritten two steps later
+      float dm = 0.0f, ym = 0.0f;
 #pragma unroll
       for (int k = 0; k < NT; ++k) {
-        dm = fmax(dm, sh.red[0][k]);
-        ym = fmax(ym, sh.red[1][k]);
+        dm = fmaxf(dm, sh.redf[it & 1][0][k]);
+        ym = fmaxf(ym, sh.redf[it & 1][1][k]);
       }
-      const double ratio = dm / ym;
-      __syncthreads();  // red[] is rewritten by the next step
+      const double ratio = (double)dm / (double)ym;
       // Stop when converged: the correction is at FP64 rounding, or the next
       // one, extrapolated from the observed contraction ratio / last_ratio, is;
       // or when stagnating at the FP64 residual's own rounding level
