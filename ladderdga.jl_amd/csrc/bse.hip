@@ -146,7 +146,8 @@ __global__ __launch_bounds__(16 * TC) void bse_gj_f64_kernel(int N, int Nq, int 
   __shared__ double red[NW];
 
   const int t = threadIdx.x, tr = t & 15, tc = t >> 4, wave = t >> 6, tcl = (t >> 4) & 3;
-  const int q = blockIdx.x % Nq, w = blockIdx.x / Nq;
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int q = lb % Nq, w = lb / Nq;
   const double2 *G = Gam + (size_t)w * N * N;
 
   // B = A^T: B[i][j] = A[j][i] = Gamma[j, i, w] (+ 1/chi0 on the diagonal); column N = 1

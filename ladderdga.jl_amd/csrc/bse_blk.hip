@@ -234,7 +234,8 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
   __shared__ BlkShared<T, NT> sh;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, n = lane & 15;
-  const int q = blockIdx.x % Nq, w = blockIdx.x / Nq;
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int q = lb % Nq, w = lb / Nq;
   const double2 *G = Gam + (size_t)w * N * N;
 
   for (int i = tid; i < NC; i += 64 * NT) {

@@ -48,6 +48,16 @@ struct OctTables {
   int stride[3];             // octant strides 1, n, n^2
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs of the GPU (block b
+// runs on XCD b % 8), each XCD with its own L2.  This maps the k-th block of
+// XCD x to a contiguous range of logical indices, so that the blocks one XCD
+// runs together work on neighbouring (q, w) points and share L2 lines of the
+// q-fastest arrays they read and write.
+__device__ __forceinline__ int xcd_logical_block(int b, int nblocks) {
+  const int x = b & 7, k = b >> 3, per = nblocks >> 3, rem = nblocks & 7;
+  return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
+}
+
 struct DevBuf {
   void *ptr = nullptr;
   size_t bytes = 0;
