@@ -395,7 +395,7 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
     constexpr bool pre = decltype(PRE)::value;
     __builtin_amdgcn_s_setprio(3);
     const int sb = kp1 % 3;
-    wave_sync_lds();
+    if constexpr (!pre) wave_sync_lds();  // own staging just before (later: a barrier in between)
     // the panel's rows in the staged copy: element j1 of the four lane groups
     const int pr0 = MT::trow(0, j1), pr1 = MT::trow(1, j1), pr2 = MT::trow(2, j1), pr3 = MT::trow(3, j1);
     auto prow = [&](int r) { return r == 0 ? pr0 : r == 1 ? pr1 : r == 2 ? pr2 : pr3; };
@@ -465,12 +465,14 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
         const unsigned k = (__float_as_uint(v) & 0xffffff00u) | (unsigned)(255 - (lane + 64 * u));
         if (!((pmask >> u) & 1u)) key = max(key, k);
       }
-      key = max(key, (unsigned)dpp32<0xB1, 0xf>((int)key, (int)key));
-      key = max(key, (unsigned)dpp32<0x4E, 0xf>((int)key, (int)key));
-      key = max(key, (unsigned)dpp32<0x141, 0xf>((int)key, (int)key));
-      key = max(key, (unsigned)dpp32<0x140, 0xf>((int)key, (int)key));
-      key = max(key, (unsigned)dpp32<0x142, 0xa>((int)key, (int)key));  // row_bcast:15 into rows 1, 3
-      key = max(key, (unsigned)dpp32<0x143, 0xc>((int)key, (int)key));  // row_bcast:31 into rows 2, 3
+      // (max with the DPP operand in one instruction; s_nop 1 covers the
+      // two wait states between a VALU write and a DPP read of the register)
+      __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf" : "+v"(key));
+      __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf" : "+v"(key));
+      __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 row_half_mirror row_mask:0xf bank_mask:0xf" : "+v"(key));
+      __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 row_mirror row_mask:0xf bank_mask:0xf" : "+v"(key));
+      __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf" : "+v"(key));
+      __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf" : "+v"(key));
       const int p = 255 - (int)(__builtin_amdgcn_readlane((int)key, 63) & 0xff);
       P[st] = p;
       const int up = p >> 6, lp = p & 63;
