@@ -117,6 +117,27 @@ __device__ __forceinline__ T row_sum16(T v) {
   v += dppv<0x140, 0xf>(v, v);  // row mirror
   return v;
 }
+// FP32: each stage is one DPP add (s_nop 1 covers the two wait states between
+// a VALU write and a DPP read of the register); same additions, same order
+template <>
+__device__ __forceinline__ float row_sum16<float>(float v) {
+  __asm__("s_nop 1\n\tv_add_f32_dpp %0, %0, %0 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf" : "+v"(v));
+  __asm__("s_nop 1\n\tv_add_f32_dpp %0, %0, %0 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf" : "+v"(v));
+  __asm__("s_nop 1\n\tv_add_f32_dpp %0, %0, %0 row_half_mirror row_mask:0xf bank_mask:0xf" : "+v"(v));
+  __asm__("s_nop 1\n\tv_add_f32_dpp %0, %0, %0 row_mirror row_mask:0xf bank_mask:0xf" : "+v"(v));
+  return v;
+}
+
+// maximum over the 64 lanes of a wave, in lane 63 (unsigned keys)
+__device__ __forceinline__ unsigned wave_max_u32(unsigned k) {
+  __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf" : "+v"(k));
+  __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf" : "+v"(k));
+  __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 row_half_mirror row_mask:0xf bank_mask:0xf" : "+v"(k));
+  __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 row_mirror row_mask:0xf bank_mask:0xf" : "+v"(k));
+  __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf" : "+v"(k));
+  __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf" : "+v"(k));
+  return k;
+}
 
 template <typename T>
 __device__ __forceinline__ typename mfma_traits<T>::c2 mkc(T x, T y) {
@@ -465,14 +486,7 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
         const unsigned k = (__float_as_uint(v) & 0xffffff00u) | (unsigned)(255 - (lane + 64 * u));
         if (!((pmask >> u) & 1u)) key = max(key, k);
       }
-      // (max with the DPP operand in one instruction; s_nop 1 covers the
-      // two wait states between a VALU write and a DPP read of the register)
-      __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf" : "+v"(key));
-      __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf" : "+v"(key));
-      __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 row_half_mirror row_mask:0xf bank_mask:0xf" : "+v"(key));
-      __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 row_mirror row_mask:0xf bank_mask:0xf" : "+v"(key));
-      __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf" : "+v"(key));
-      __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf" : "+v"(key));
+      key = wave_max_u32(key);
       const int p = 255 - (int)(__builtin_amdgcn_readlane((int)key, 63) & 0xff);
       P[st] = p;
       const int up = p >> 6, lp = p & 63;
@@ -670,19 +684,7 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
       }
       // wave maxima by DPP (non-negative floats order like their bit patterns;
       // a NaN stays the maximum), then across the waves through LDS
-      unsigned dk = __float_as_uint(dmax), yk = __float_as_uint(ymax);
-      dk = max(dk, (unsigned)dpp32<0xB1, 0xf>((int)dk, (int)dk));
-      yk = max(yk, (unsigned)dpp32<0xB1, 0xf>((int)yk, (int)yk));
-      dk = max(dk, (unsigned)dpp32<0x4E, 0xf>((int)dk, (int)dk));
-      yk = max(yk, (unsigned)dpp32<0x4E, 0xf>((int)yk, (int)yk));
-      dk = max(dk, (unsigned)dpp32<0x141, 0xf>((int)dk, (int)dk));
-      yk = max(yk, (unsigned)dpp32<0x141, 0xf>((int)yk, (int)yk));
-      dk = max(dk, (unsigned)dpp32<0x140, 0xf>((int)dk, (int)dk));
-      yk = max(yk, (unsigned)dpp32<0x140, 0xf>((int)yk, (int)yk));
-      dk = max(dk, (unsigned)dpp32<0x142, 0xa>((int)dk, (int)dk));
-      yk = max(yk, (unsigned)dpp32<0x142, 0xa>((int)yk, (int)yk));
-      dk = max(dk, (unsigned)dpp32<0x143, 0xc>((int)dk, (int)dk));
-      yk = max(yk, (unsigned)dpp32<0x143, 0xc>((int)yk, (int)yk));
+      const unsigned dk = wave_max_u32(__float_as_uint(dmax)), yk = wave_max_u32(__float_as_uint(ymax));
       if (lane == 63) {
         sh.redf[it & 1][0][wave] = __uint_as_float(dk);
         sh.redf[it & 1][1][wave] = __uint_as_float(yk);
