@@ -123,6 +123,27 @@ def test_bse_mixed_nnu160(ctx):
         L.calc_χγ("m", inp.Γ_m, χ0, inp.kG, inp.mP, inp.sP, precision="fp64", ctx=ctx)
 
 
+@pytest.mark.parametrize("n_iν", [13, 28, 36, 44, 52, 59, 60, 68])
+def test_bse_every_tile_count(ctx, n_iν):
+    """Every kernel instance NT = 3..9 (NT = max(3, ⌈Nν/16⌉); Nν = 26 and 118 leave a partial last
+    tile) against LAPACK, both channels, FP64 up to Nν = 128 and mixed."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP = setup("tiny", D=3, Ns=4, n_iν=n_iν, n_iω=1, U=1.5, β=12.0, n=1.0, t=0.408248290463863)
+    Nν = 2 * n_iν
+    print(f"bse tile count Nν={Nν}", flush=True)
+    ref_χ0 = O.calc_bubble(inp.G_fft, inp.G_rfft, oK, omP, osP)
+    χ0 = L.χ0T(np.asfortranarray(ref_χ0), inp.mP.β)
+    precs = ["fp64", "mixed"] if Nν <= 128 else ["mixed"]
+    for ch, Γ in (("m", inp.Γ_m), ("d", inp.Γ_d)):
+        rχ, rγ = O.calc_χγ(ch, Γ, ref_χ0, oK, omP, osP)
+        for prec in precs:
+            rep = {}
+            χ, γ = L.calc_χγ(ch, Γ, χ0, inp.kG, inp.mP, inp.sP, precision=prec, ctx=ctx, report=rep)
+            assert relerr(χ.data, rχ) <= TOL and relerr(γ.data, rγ) <= TOL, (ch, prec)
+            if prec == "mixed":
+                assert rep["n_fallback"] == 0
+
+
 def test_bse_zero_pivot_reports_and_gives_nan(ctx):
     """A = Γ + diag(1/χ₀) exactly singular -> LDGA_ERR_ZERO_PIVOT status and NaN outputs."""
     import ctypes as C
