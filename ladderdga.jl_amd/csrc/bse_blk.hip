@@ -13,16 +13,18 @@
 // y = inv(M) 1, is y[p_r] = sum_c M[r][c] (a row sum of the stored matrix).
 //
 // Layout: the padded NC x NC matrix (NC = 16 NT, identity on the padding)
-// lives in MFMA accumulators.  Wave w owns rows 16w..16w+15 as NT tiles of
-// 16x16 (real and imaginary planes); lane (g = lane/16, n = lane%16) holds
-// column 16t+n of tile t and the four rows trow(g, j), j = 0..3, in element j.
+// lives in MFMA accumulators.  Wave w holds 16 rows as NT tiles of 16x16
+// (real and imaginary planes); lane (g = lane/16, n = lane%16) holds column
+// 16t+n of tile t and, in element j, matrix row 4 (j NT + w) + g (tile row
+// trow(g, j)).  The row order is free: the elimination tracks which column
+// each row pivots on.
 //
 // Rows are processed in panels of four: panel (j, w) = element j of every
-// lane group of wave w.  The owner wave copies the panel to LDS, reloads it
-// one column per lane (columns lane + 64u), runs the four pivot steps there
-// (64-lane DPP argmax, readlane broadcasts, no workgroup barrier), and
-// publishes W = (panel after) - E_P.  After one barrier every wave applies
-// the panel to its own rows as a rank-4 block update on MFMA,
+// lane group of wave w.  The owner wave stages its rows in LDS, loads the
+// panel rows one column per lane (columns lane + 64u), runs the four pivot
+// steps there (64-lane DPP argmax, readlane broadcasts, no workgroup
+// barrier), and publishes W = (panel after) - E_P.  After one barrier every
+// wave applies the panel to its own rows as a rank-4 block update on MFMA,
 //     M[S,:] += M[S,P] W     (S = rows outside the panel, M[S,P] before),
 // the complex product split into four real K=4 products (4M scheme).  The
 // panel ownership rotates over the waves.
@@ -86,10 +88,21 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
   const int q = lb % Nq, w = lb / Nq;
   const double2 *G = Gam + (size_t)w * N * N;
 
+  // Matrix row i is stored at accumulator element j of lane group g of wave wv
+  // with i = 4 (j NT + wv) + g, so that panel kp (element kp / NT of wave
+  // kp % NT) holds rows 4kp .. 4kp + 3: the padding rows come last and the
+  // panels holding only padding rows (identity rows, whose updates are zero)
+  // are skipped.
+  const int NPe = (N + 3) / 4;  // panels processed (of 4 NT)
+  auto pos_of_row = [&](int i) {
+    const int kp = i >> 2;
+    return 16 * (kp % NT) + MT::trow(i & 3, kp / NT);
+  };
   for (int i = tid; i < NC; i += 64 * NT) {
     double2 d = cz(0.0, 0.0);
     if (i < N) d = crecip(chi0[q + (size_t)Nq * (shell + i + (size_t)Nc * w)]);
     sh.dinv[i] = d;
+    sh.pcol_of_row[pos_of_row(i)] = i;  // the skipped identity rows keep their own column
   }
   if (tid == 0) sh.zero_pivot = 0;
   __syncthreads();
@@ -114,7 +127,7 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
       const int c = 16 * (t0 + tt) + nl;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int i = 16 * wave + MT::trow(gl, j);
+        const int i = 4 * (j * NT + wave) + gl;
         const bool ok = t0 + tt < NT && i < N && c < N;
         v[tt][j] = G[ok ? c + N * i : 0];
         if (!ok) v[tt][j] = cz(0.0, 0.0);
@@ -124,12 +137,11 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
     for (int tt = 0; tt < LG; ++tt) {
       const int t = t0 + tt;
       if (t < NT) {
-        if (t == wave) {  // the diagonal tile (uniform branch)
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const int i = 16 * wave + MT::trow(g, j);
-            if (n == MT::trow(g, j)) v[tt][j] = cadd(v[tt][j], i < N ? sh.dinv[i] : cz(1.0, 0.0));
-          }
+        for (int j = 0; j < 4; ++j) {  // the diagonal: + 1/chi0[i], or 1 on the padding
+          const int i = 4 * (j * NT + wave) + g;
+          const double2 d = i < N ? sh.dinv[i] : cz(1.0, 0.0);
+          if (16 * t + n == i) v[tt][j] = cadd(v[tt][j], d);
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -141,10 +153,10 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
     }
     __asm__ volatile("" ::: "memory");
   }
-  unsigned pmask = 0;  // pivoted columns lane + 64u, bit u (panel layout)
+  unsigned pmask = 0;  // pivoted columns lane + 64u, bit u (panel layout); the skipped rows' too
 #pragma unroll
   for (int u = 0; u < NU; ++u)
-    if (lane + 64 * u >= NC) pmask |= 1u << u;
+    if (lane + 64 * u >= 4 * NPe) pmask |= 1u << u;
   int zero_seen = 0;
 
   // ---- panels: a two-stage software pipeline -----------------------------------
@@ -160,7 +172,6 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
   // read up to iteration kp+1 and rewritten by the owner of panel kp+3 after
   // the barrier of iteration kp+2); the staging buffer is double-buffered and
   // written and read by the same wave.
-  constexpr int NP = 4 * NT;
   static_assert(NT >= 2, "the pipeline needs two waves");
   // Per-lane parts of the LDS addresses, re-laundered every iteration: the
   // compiler then forms each address as base + immediate offset instead of
@@ -389,7 +400,7 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
   if (wave == 0) stage_rows(0);
   if (wave == 1) stage_rows(1);
   if (wave == 0) factor(std::false_type{}, 0, 0, 0, 0, 0, 0, 0);
-  for (int kp = 0; kp < NP; ++kp) {
+  for (int kp = 0; kp < NPe; ++kp) {
     const int o = kp % NT, j = kp / NT, bw = kp % 3;
     const int kp1 = kp + 1, o1 = kp1 % NT;
     __asm__ volatile("" : "+v"(Lg), "+v"(Ln), "+v"(Ll));
@@ -402,7 +413,7 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
       for (int st = 0; st < 4; ++st)
         if ((Ps[st] & 63) == lane) pmask |= 1u << (Ps[st] >> 6);
     }
-    if (kp1 < NP && wave == o1) {
+    if (kp1 < NPe && wave == o1) {
       // the critical path: straight from the barrier to the next barrier; this
       // wave's own MFMA update by W_kp is applied in the next iteration
       factor(std::true_type{}, kp1, kp1 / NT, bw, Ps[0], Ps[1], Ps[2], Ps[3]);
@@ -429,7 +440,7 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
         else with_j(j, body);
       }
       // the owner of panel kp+2 stages its rows (updated through panel kp)
-      if (kp + 2 < NP && wave == (kp + 2) % NT) stage_rows(kp + 2);
+      if (kp + 2 < NPe && wave == (kp + 2) % NT) stage_rows(kp + 2);
     }
     __builtin_amdgcn_wave_barrier();
   }
@@ -481,7 +492,7 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
         if (n == 0 && i < N) {
           const double2 yi = sh.y[i];
           const double2 d = cmul(yi, sh.dinv[i]);
-          sh.res[sh.pcol_of_row[i]] = cz(1.0 - acc.x - d.x, -acc.y - d.y);
+          sh.res[sh.pcol_of_row[pos_of_row(i)]] = cz(1.0 - acc.x - d.x, -acc.y - d.y);
         }
       }
       if (tid < NC - N) sh.res[N + tid] = cz(0.0, 0.0);
