@@ -123,10 +123,11 @@ def test_bse_mixed_nnu160(ctx):
         L.calc_χγ("m", inp.Γ_m, χ0, inp.kG, inp.mP, inp.sP, precision="fp64", ctx=ctx)
 
 
-@pytest.mark.parametrize("n_iν", [13, 28, 36, 44, 52, 59, 60, 68])
+@pytest.mark.parametrize("n_iν", [1, 2, 13, 28, 36, 44, 52, 59, 60, 68])
 def test_bse_every_tile_count(ctx, n_iν):
     """Every kernel instance NT = 3..9 (NT = max(3, ⌈Nν/16⌉); Nν = 26 and 118 leave a partial last
-    tile) against LAPACK, both channels, FP64 up to Nν = 128 and mixed."""
+    tile and a partial last panel; Nν = 2 and 4 process a single panel) against LAPACK, both
+    channels, FP64 up to Nν = 128 and mixed."""
     import ladderdga_hip as L
     inp, oK, omP, osP = setup("tiny", D=3, Ns=4, n_iν=n_iν, n_iω=1, U=1.5, β=12.0, n=1.0, t=0.408248290463863)
     Nν = 2 * n_iν
