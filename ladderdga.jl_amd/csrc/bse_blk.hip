@@ -32,8 +32,10 @@
 // T = double: the FP64 path (f64 16x16x4 MFMA).  T = float with iterative
 // refinement: the mixed path -- the FP32 inverse X is kept in the
 // accumulators and y is refined in FP64:  r = 1 - A^T y  (Gamma, chi0 in FP64),
-// y += X r, until |X r|_inf <= 2^-50 |y|_inf or the correction stops halving
-// (the FP64 residual's rounding level, kappa(A) 2^-53, is reached).
+// y += X r, until |X r|_inf <= 2^-50 |y|_inf (or the next correction,
+// extrapolated from the contraction rate, is at FP64 rounding) or the
+// correction stops halving (the FP64 residual's rounding level, kappa(A)
+// 2^-53, is reached).
 #include <climits>
 #include <cstring>
 #include <type_traits>
@@ -55,7 +57,7 @@ struct BlkShared {
   T Cr[NT][16][4];     // per wave: own rows at the four pivot columns (real)
   T Ci[NT][16][4];     // (imaginary)
   int P[3][4];         // pivot columns of the panel
-  int pcol_of_row[NC]; // pivot column chosen for each row
+  int pcol_of_row[NC]; // pivot column of the row at each accumulator position
   double2 y[NC];       // solution (FP64)
   double2 dinv[NC];    // 1/chi0 on the diagonal (FP64)
   double2 res[NC];     // refinement residual (FP64), at the pivot column of its row
