@@ -69,9 +69,13 @@ typedef struct {
   int64_t n_matrices;    /* matrices factorised                              */
   int64_t n_zero_pivot;  /* matrices with an exactly zero pivot              */
   int32_t ir_steps_max;  /* mixed mode: largest refinement step count used   */
-  int32_t n_fallback;    /* mixed mode: matrices whose refinement ended above 2^-30 */
+  int32_t n_fallback;    /* mixed mode: matrices whose refinement ended above 2^-30
+                            (or met a zero FP32 pivot)                       */
   double max_rel_resid;  /* mixed mode: largest final correction |dy|/|y|    */
   double ir_steps_mean;  /* mixed mode: mean refinement step count           */
+  int64_t n_fp64_resolved; /* mixed mode: of n_fallback, matrices re-solved by the
+                            FP64 kernel (all of them when Nnu <= 128; their chi
+                            and gamma are the FP64 results)                  */
 } ldga_bse_report;
 
 /* ---- context --------------------------------------------------------- */

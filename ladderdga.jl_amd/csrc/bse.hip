@@ -269,10 +269,10 @@ int run_gj(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double
 
 int launch_bse_mixed(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam, const double2 *chi0,
                      double s, double U, double beta, double *chi, double2 *gamma, unsigned long long *counters,
-                     hipStream_t st);
+                     int *redo, hipStream_t st);
 int launch_bse_blk_f64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam,
                        const double2 *chi0, double s, double U, double beta, double *chi, double2 *gamma,
-                       unsigned long long *counters, hipStream_t st);
+                       unsigned long long *counters, const int *only, int n_only, hipStream_t st);
 
 // The register-tile Gauss-Jordan kernel above (Nnu <= 127) is kept as the
 // second FP64 implementation; LDGA_BSE_F64=gj selects it instead of the
@@ -287,14 +287,20 @@ int launch_bse(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double 
   if (s != 1 && s != -1) return ldga_fail(ctx, LDGA_ERR_ARG, "channel sign s must be +1 (m) or -1 (d)");
   const int N = 2 * fr->n_inu, Nw = 2 * fr->n_iw + 1, Nq = ctx->Nq, Nc = N + 2 * fr->n_inu_shell;
   // counters: [0] zero pivots, [1] refinement steps max, [2] unconverged refinements,
-  // [3] largest final refinement ratio (bits), [4] pipeline errors of the blocked kernel,
-  // [5] refinement steps summed over the matrices
-  unsigned long long *cnt = (unsigned long long *)ldga_buf(ctx, "bse_counters", 8 * 8);
+  // [3] largest final refinement ratio (bits), [5] refinement steps summed over the
+  // matrices; [8..15]: the same for the FP64 re-solve
+  unsigned long long *cnt = (unsigned long long *)ldga_buf(ctx, "bse_counters", 16 * 8);
   if (!cnt) return LDGA_ERR_HIP;
-  LDGA_HIP_TRY(ctx, hipMemsetAsync(cnt, 0, 8 * 8, st));
+  LDGA_HIP_TRY(ctx, hipMemsetAsync(cnt, 0, 16 * 8, st));
+  // mixed mode with an FP64 kernel for this size: the refinement's failures
+  // are listed (room for every matrix) and re-solved in FP64
+  const bool resolve = precision == LDGA_PREC_MIXED && N <= 128;
+  int *redo = nullptr;
+  if (resolve && !(redo = (int *)ldga_buf(ctx, "bse_redo", (size_t)Nq * Nw * sizeof(int)))) return LDGA_ERR_HIP;
   int rc;
   if (precision == LDGA_PREC_FP64 && !use_register_gj()) {
-    rc = launch_bse_blk_f64(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, st);
+    rc = launch_bse_blk_f64(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, nullptr, 0,
+                            st);
   } else if (precision == LDGA_PREC_FP64) {
     if (N <= 47)
       rc = run_gj<3, 3, 16>(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, st);
@@ -307,25 +313,36 @@ int launch_bse(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double 
     else
       return ldga_fail(ctx, LDGA_ERR_SHAPE, "FP64 BSE supports Nnu <= 127 (got %d)", N);
   } else if (precision == LDGA_PREC_MIXED) {
-    rc = launch_bse_mixed(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, st);
+    rc = launch_bse_mixed(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, redo, st);
   } else {
     return ldga_fail(ctx, LDGA_ERR_ARG, "unknown precision mode %d", precision);
   }
   if (rc) return rc;
-  if (rep) {
-    unsigned long long h[6];
+  unsigned long long h[6] = {0, 0, 0, 0, 0, 0}, h2[6] = {0, 0, 0, 0, 0, 0};
+  if (rep || resolve) {
     LDGA_HIP_TRY(ctx, hipMemcpyAsync(h, cnt, sizeof h, hipMemcpyDeviceToHost, st));
     LDGA_HIP_TRY(ctx, hipStreamSynchronize(st));
+  }
+  if (resolve && h[2]) {
+    // the listed matrices' chi and gamma are overwritten by the FP64 kernel
+    rc = launch_bse_blk_f64(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt + 8, redo,
+                            (int)h[2], st);
+    if (rc) return rc;
+    LDGA_HIP_TRY(ctx, hipMemcpyAsync(h2, cnt + 8, sizeof h2, hipMemcpyDeviceToHost, st));
+    LDGA_HIP_TRY(ctx, hipStreamSynchronize(st));
+  }
+  const unsigned long long nzero = h[0] + h2[0];
+  if (rep) {
     rep->n_matrices = (int64_t)Nq * Nw;
-    rep->n_zero_pivot = (int64_t)h[0];
+    rep->n_zero_pivot = (int64_t)nzero;
     rep->ir_steps_max = (int32_t)h[1];
     rep->n_fallback = (int32_t)h[2];
     double r;
     memcpy(&r, &h[3], 8);
     rep->max_rel_resid = r;
     rep->ir_steps_mean = (double)h[5] / (double)((int64_t)Nq * Nw);
-    if (h[4]) return ldga_fail(ctx, LDGA_ERR_STATE, "%llu BSE workgroups hit the staging wait bound", h[4]);
-    if (h[0]) return ldga_fail(ctx, LDGA_ERR_ZERO_PIVOT, "%llu BSE matrices had an exactly zero pivot", h[0]);
+    rep->n_fp64_resolved = resolve ? (int64_t)h[2] : 0;
   }
+  if (nzero) return ldga_fail(ctx, LDGA_ERR_ZERO_PIVOT, "%llu BSE matrices had an exactly zero pivot", nzero);
   return LDGA_OK;
 }

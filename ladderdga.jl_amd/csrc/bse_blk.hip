@@ -78,7 +78,8 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
                                                           const double2 *__restrict__ chi0, double s, double U,
                                                           double beta, int max_ir, double *__restrict__ chi_out,
                                                           double2 *__restrict__ gamma_out,
-                                                          unsigned long long *__restrict__ counters) {
+                                                          unsigned long long *__restrict__ counters,
+                                                          int *__restrict__ redo, const int *__restrict__ only) {
   typedef mfma_traits<T> MT;
   typedef typename MT::v4 v4;
   typedef typename MT::c2 c2;
@@ -86,7 +87,9 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
   __shared__ BlkShared<T, NT> sh;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, n = lane & 15;
-  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  // only != NULL: block b solves matrix only[b] (the FP64 re-solve of a
+  // refinement's failures); otherwise every (q, w), XCD-aware
+  const int lb = only ? only[blockIdx.x] : xcd_logical_block(blockIdx.x, gridDim.x);
   const int q = lb % Nq, w = lb / Nq;
   const double2 *G = Gam + (size_t)w * N * N;
 
@@ -553,7 +556,13 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
     if (tid == 0) {
       atomicMax(&counters[1], (unsigned long long)it);
       atomicAdd(&counters[5], (unsigned long long)it);
-      if (!(last_ratio <= 0x1p-30)) atomicAdd(&counters[2], 1ull);
+      // a refinement that ended above 2^-30 (or met a zero FP32 pivot) is
+      // listed for the FP64 re-solve when the caller provides the list (room
+      // for every matrix)
+      if (!(last_ratio <= 0x1p-30) || sh.zero_pivot) {
+        const unsigned long long k = atomicAdd(&counters[2], 1ull);
+        if (redo) redo[k] = lb;
+      }
       unsigned long long bits;
       const double lr = last_ratio == last_ratio ? last_ratio : 1.0;
       memcpy(&bits, &lr, 8);
@@ -578,57 +587,77 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
   }
   if (tid == 0) {
     chi_out[q + (size_t)Nq * w] = chi;
-    if (sh.zero_pivot) atomicAdd(&counters[0], 1ull);
+    // a listed matrix's zero pivot is decided by its FP64 re-solve
+    if (sh.zero_pivot && !(REFINE && redo)) atomicAdd(&counters[0], 1ull);
   }
 }
 
 template <typename T, int NT, bool REFINE>
 int run_blk(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam, const double2 *chi0,
             double s, double U, double beta, int max_ir, double *chi, double2 *gamma, unsigned long long *cnt,
-            hipStream_t st) {
+            int *redo, const int *only, int n_only, hipStream_t st) {
   if (N > 16 * NT) return ldga_fail(ctx, LDGA_ERR_SHAPE, "BSE tile too small for N=%d", N);
-  bse_blk_kernel<T, NT, REFINE><<<dim3((unsigned)Nq * Nw), dim3(64 * NT), 0, st>>>(
-      N, Nq, Nc, shell, Gam, chi0, s, U, beta, max_ir, chi, gamma, cnt);
+  const unsigned nblocks = only ? (unsigned)n_only : (unsigned)Nq * Nw;
+  if (nblocks == 0) return LDGA_OK;
+  bse_blk_kernel<T, NT, REFINE><<<dim3(nblocks), dim3(64 * NT), 0, st>>>(N, Nq, Nc, shell, Gam, chi0, s, U, beta,
+                                                                         max_ir, chi, gamma, cnt, redo, only);
   LDGA_HIP_TRY(ctx, hipGetLastError());
   return LDGA_OK;
 }
 
 }  // namespace
 
-// FP64 blocked kernel: Nnu <= 128.
+// FP64 blocked kernel: Nnu <= 128.  With only != NULL it solves just the
+// n_only matrices (q + Nq w) listed there.
 int launch_bse_blk_f64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam,
                        const double2 *chi0, double s, double U, double beta, double *chi, double2 *gamma,
-                       unsigned long long *cnt, hipStream_t st) {
+                       unsigned long long *cnt, const int *only, int n_only, hipStream_t st) {
   const int nt = (N + 15) / 16;
   switch (nt) {
     case 1:
     case 2:
-    case 3: return run_blk<double, 3, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, st);
-    case 4: return run_blk<double, 4, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, st);
-    case 5: return run_blk<double, 5, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, st);
-    case 6: return run_blk<double, 6, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, st);
-    case 7: return run_blk<double, 7, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, st);
-    case 8: return run_blk<double, 8, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, st);
+    case 3: return run_blk<double, 3, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, nullptr,
+                                                only, n_only, st);
+    case 4: return run_blk<double, 4, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, nullptr,
+                                                only, n_only, st);
+    case 5: return run_blk<double, 5, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, nullptr,
+                                                only, n_only, st);
+    case 6: return run_blk<double, 6, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, nullptr,
+                                                only, n_only, st);
+    case 7: return run_blk<double, 7, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, nullptr,
+                                                only, n_only, st);
+    case 8: return run_blk<double, 8, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, nullptr,
+                                                only, n_only, st);
     default: return ldga_fail(ctx, LDGA_ERR_SHAPE, "FP64 blocked BSE supports Nnu <= 128 (got %d)", N);
   }
 }
 
-// Mixed: FP32 inverse on MFMA + FP64 refinement, Nnu <= 160.
+// Mixed: FP32 inverse on MFMA + FP64 refinement, Nnu <= 160.  With redo !=
+// NULL (room for Nq Nw ints) the matrices whose refinement failed are listed
+// there; counters[2] holds their number.
 int launch_bse_mixed(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam, const double2 *chi0,
                      double s, double U, double beta, double *chi, double2 *gamma, unsigned long long *cnt,
-                     hipStream_t st) {
+                     int *redo, hipStream_t st) {
   const int nt = (N + 15) / 16, ir = 6;
   switch (nt) {
     case 1:
     case 2:
-    case 3: return run_blk<float, 3, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, st);
-    case 4: return run_blk<float, 4, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, st);
-    case 5: return run_blk<float, 5, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, st);
-    case 6: return run_blk<float, 6, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, st);
-    case 7: return run_blk<float, 7, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, st);
-    case 8: return run_blk<float, 8, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, st);
-    case 9: return run_blk<float, 9, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, st);
-    case 10: return run_blk<float, 10, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, st);
+    case 3: return run_blk<float, 3, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo,
+                                                nullptr, 0, st);
+    case 4: return run_blk<float, 4, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo,
+                                                nullptr, 0, st);
+    case 5: return run_blk<float, 5, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo,
+                                                nullptr, 0, st);
+    case 6: return run_blk<float, 6, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo,
+                                                nullptr, 0, st);
+    case 7: return run_blk<float, 7, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo,
+                                                nullptr, 0, st);
+    case 8: return run_blk<float, 8, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo,
+                                                nullptr, 0, st);
+    case 9: return run_blk<float, 9, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo,
+                                                nullptr, 0, st);
+    case 10: return run_blk<float, 10, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo,
+                                                nullptr, 0, st);
     default: return ldga_fail(ctx, LDGA_ERR_SHAPE, "mixed BSE supports Nnu <= 160 (got %d)", N);
   }
 }

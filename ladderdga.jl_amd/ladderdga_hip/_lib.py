@@ -51,7 +51,8 @@ class Model(C.Structure):
 
 class BseReport(C.Structure):
     _fields_ = [("n_matrices", C.c_int64), ("n_zero_pivot", C.c_int64), ("ir_steps_max", C.c_int32),
-                ("n_fallback", C.c_int32), ("max_rel_resid", C.c_double), ("ir_steps_mean", C.c_double)]
+                ("n_fallback", C.c_int32), ("max_rel_resid", C.c_double), ("ir_steps_mean", C.c_double),
+                ("n_fp64_resolved", C.c_int64)]
 
 
 class LambdaDmOut(C.Structure):

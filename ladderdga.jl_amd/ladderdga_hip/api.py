@@ -73,12 +73,14 @@ def calc_χγ(type_: str, Γr, χ0: χ0T, kG: KGrid, mP: ModelParameters, sP: Si
     if rc == _lib.LDGA_ERR_ZERO_PIVOT:
         # the reference ignores LAPACK's info (src/LapackWrapper.jl:31); keep going with NaN
         warnings.warn(f"{rep.n_zero_pivot} BSE matrices are exactly singular; their χ, γ are NaN")
-    if precision == "mixed" and rep.n_fallback:
-        warnings.warn(f"{rep.n_fallback} BSE matrices did not reach FP64 accuracy by iterative refinement "
-                      f"(largest final correction {rep.max_rel_resid:.3g} of |y|)")
+    if precision == "mixed" and rep.n_fallback > rep.n_fp64_resolved:
+        # refinement failures are re-solved in FP64 where an FP64 kernel exists (Nν <= 128)
+        warnings.warn(f"{rep.n_fallback - rep.n_fp64_resolved} BSE matrices did not reach FP64 accuracy by "
+                      f"iterative refinement (largest final correction {rep.max_rel_resid:.3g} of |y|)")
     if report is not None:
         report.update(n_matrices=rep.n_matrices, n_zero_pivot=rep.n_zero_pivot, ir_steps_max=rep.ir_steps_max,
-                      n_fallback=rep.n_fallback, max_rel_resid=rep.max_rel_resid, ir_steps_mean=rep.ir_steps_mean)
+                      n_fallback=rep.n_fallback, max_rel_resid=rep.max_rel_resid, ir_steps_mean=rep.ir_steps_mean,
+                      n_fp64_resolved=rep.n_fp64_resolved)
     return χT(χ, mP.β, tail_c=[0.0, 0.0, mP.Ekin_DMFT]), γT(γ)
 
 

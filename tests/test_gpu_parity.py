@@ -145,6 +145,63 @@ def test_bse_every_tile_count(ctx, n_iν):
                 assert rep["n_fallback"] == 0
 
 
+def _make_ill_conditioned(Γ, χ0, targets, κ=1e9):
+    """Γ with A(q, ω) = Γ[:,:,ω] + diag(1/χ₀[q,:,ω]) of condition number κ at each (q, ω) in targets
+    (one per ω): a rank-1 change of Γ[:,:,ω] replaces A's smallest singular value by σ_max/κ."""
+    Γ = np.array(Γ, order="F", copy=True)
+    for q, w in targets:
+        U, σ, Vh = np.linalg.svd(Γ[:, :, w] + np.diag(1.0 / χ0[q, :, w]))
+        Γ[:, :, w] += (σ[0] / κ - σ[-1]) * np.outer(U[:, -1], Vh[-1, :])
+    return Γ
+
+
+# targets whose rank-1 change leaves every other matrix of its ω at κ < 5e4 (q points related by lattice
+# symmetry share χ₀ and so their A; the test finds those through κ)
+RESOLVE_CASES = [(dict(D=2, Ns=4, n_iν=4, n_iω=3), [(0, 0)]),
+                 (dict(D=3, Ns=4, n_iν=60, n_iω=1, U=1.5, β=12.0, n=1.0, t=0.408248290463863), [(9, 2), (8, 0)]),
+                 (dict(D=3, Ns=4, n_iν=80, n_iω=1, U=2.0, β=25.0, n=1.0, t=0.408248290463863), [(9, 1)])]
+
+
+@pytest.mark.parametrize("kw,targets", RESOLVE_CASES)
+def test_bse_mixed_refinement_failure_resolved_in_fp64(ctx, kw, targets):
+    """Matrices of condition number 1e9 are beyond an FP32 inverse: their refinement fails.  Up to
+    Nν = 128 the library re-solves exactly those with the FP64 kernel, so their χ, γ equal the FP64
+    call's bit for bit; at Nν = 160 (no FP64 kernel) they are reported and the call warns.  Every other
+    (q, ω) meets the FP64 bar against LAPACK either way."""
+    import warnings
+    import ladderdga_hip as L
+    inp, oK, omP, osP = setup("tiny", **kw)
+    Nν = 2 * inp.sP.n_iν
+    print(f"bse mixed re-solve Nν={Nν} targets={targets}", flush=True)
+    ref_χ0 = O.calc_bubble(inp.G_fft, inp.G_rfft, oK, omP, osP)
+    χ0 = L.χ0T(np.asfortranarray(ref_χ0), inp.mP.β)
+    Γ = _make_ill_conditioned(inp.Γ_m, ref_χ0, targets)
+    κ = np.array([[np.linalg.cond(Γ[:, :, w] + np.diag(1.0 / ref_χ0[q, :, w])) for w in range(Γ.shape[2])]
+                  for q in range(ref_χ0.shape[0])])
+    bad = [tuple(qw) for qw in np.argwhere(κ > 1e8)]
+    assert set(targets) <= set(bad) and np.max(κ[κ <= 1e8]) < 5e4
+    rep = {}
+    with warnings.catch_warnings(record=True) as warned:
+        warnings.simplefilter("always")
+        χ, γ = L.calc_χγ("m", Γ, χ0, inp.kG, inp.mP, inp.sP, precision="mixed", ctx=ctx, report=rep)
+    rχ, rγ = O.calc_χγ("m", Γ, ref_χ0, oK, omP, osP)
+    assert rep["n_fallback"] == len(bad) and rep["n_zero_pivot"] == 0
+    if Nν <= 128:
+        assert rep["n_fp64_resolved"] == rep["n_fallback"] and not warned
+        χf, γf = L.calc_χγ("m", Γ, χ0, inp.kG, inp.mP, inp.sP, precision="fp64", ctx=ctx)
+        for q, w in bad:
+            assert χ.data[q, w] == χf.data[q, w] and np.array_equal(γ.data[q, :, w], γf.data[q, :, w])
+            # FP64 against LAPACK at κ = 1e9: both are off the exact solution by up to ~κ 2^-53 Nν
+            assert relerr(χ.data[q, w], rχ[q, w]) <= 1e-4 and relerr(γ.data[q, :, w], rγ[q, :, w]) <= 1e-4
+    else:
+        assert rep["n_fp64_resolved"] == 0 and len(warned) == 1
+    keep = np.ones(rχ.shape, dtype=bool)
+    for q, w in bad:
+        keep[q, w] = False
+    assert relerr(χ.data[keep], rχ[keep]) <= TOL
+    assert relerr(γ.data.transpose(0, 2, 1)[keep], rγ.transpose(0, 2, 1)[keep]) <= TOL
+
+
 def test_bse_zero_pivot_reports_and_gives_nan(ctx):
     """A = Γ + diag(1/χ₀) exactly singular -> LDGA_ERR_ZERO_PIVOT status and NaN outputs."""
     import ctypes as C
