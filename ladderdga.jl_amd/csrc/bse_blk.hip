@@ -33,9 +33,9 @@
 // refinement: the mixed path -- the FP32 inverse X is kept in the
 // accumulators and y is refined in FP64:  r = 1 - A^T y  (Gamma, chi0 in FP64),
 // y += X r, until |X r|_inf <= 2^-50 |y|_inf (or the next correction,
-// extrapolated from the contraction rate, is at FP64 rounding) or the
-// correction stops halving (the FP64 residual's rounding level, kappa(A)
-// 2^-53, is reached).
+// extrapolated from the contraction rate, is <= 2^-48 |y|_inf, a few units in
+// the last place) or the correction stops halving (the FP64 residual's
+// rounding level, kappa(A) 2^-53, is reached).
 #include <climits>
 #include <cstring>
 #include <type_traits>
@@ -57,10 +57,11 @@ struct BlkShared {
   T Cr[NT][16][4];     // per wave: own rows at the four pivot columns (real)
   T Ci[NT][16][4];     // (imaginary)
   int P[3][4];         // pivot columns of the panel
+  unsigned pm[(NC + 31) / 32];  // pivoted columns (bit c % 32 of word c / 32); the skipped rows' too
   int pcol_of_row[NC]; // pivot column of the row at each accumulator position
   double2 y[NC];       // solution (FP64)
   double2 dinv[NC];    // 1/chi0 on the diagonal (FP64)
-  double2 res[NC];     // refinement residual (FP64), at the pivot column of its row
+  c2 res[NC];          // refinement residual (rounded to T for X r), at the pivot column of its row
   double red[2][NT];
   float redf[2][2][NT];
   int zero_pivot;
@@ -110,58 +111,73 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
     sh.pcol_of_row[pos_of_row(i)] = i;  // the skipped identity rows keep their own column
   }
   if (tid == 0) sh.zero_pivot = 0;
+  // the columns of the skipped padding rows (>= 4 NPe) never pivot
+  for (int k = tid; k < (NC + 31) / 32; k += 64 * NT) {
+    const int lo = 4 * NPe - 32 * k;
+    sh.pm[k] = lo <= 0 ? ~0u : lo >= 32 ? 0u : ~0u << lo;
+  }
   __syncthreads();
 
   // ---- M = A^T into the accumulators: M[i][c] = Gamma[c, i, w] + delta_ic / chi0[i]
   // (identity on the padding).  The loads carry no branches, so the loads of a
   // group of LG tiles are in flight together; groups are issued one after the
-  // other to bound the registers they occupy.
-  AccPlane<T, NT> Vr, Vi;
-  Vr.zero();
-  Vi.zero();
+  // other to bound the registers they occupy.  Rows and columns past N - 1 are
+  // clamped for the load (a valid address) and zeroed after the conversion.
+  // Row i's diagonal lies in tile (j NT + wave) / 4, the same for the whole
+  // wave, at lane i % 16: it is added there only, under a scalar branch.
+  AccPlane<T, NT> Vr, Vi;  // every element is set below
   constexpr int LG = sizeof(T) == 4 ? 2 : 4;
+  const char *Gb = reinterpret_cast<const char *>(G);
+  int tdiag[4];
+  unsigned rofs[4];  // byte offset of row i (clamped) in Gamma(w)
+  T dgx[4], dgy[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int i = 4 * (j * NT + wave) + g;
+    rofs[j] = 16u * (unsigned)N * (unsigned)min(i, N - 1);
+    tdiag[j] = __builtin_amdgcn_readfirstlane((j * NT + wave) >> 2);
+    const double2 d = i < N ? sh.dinv[i] : cz(1.0, 0.0);  // 1/chi0[i], or 1 on the padding
+    const bool here = n == (i & 15);
+    dgx[j] = here ? (T)d.x : (T)0;
+    dgy[j] = here ? (T)d.y : (T)0;
+  }
 #pragma unroll
   for (int t0 = 0; t0 < NT; t0 += LG) {
-    // the lane indices are laundered per group so that no address of a later
+    // the lane index is laundered per group so that no address of a later
     // group is formed (and held in registers) early
-    int nl = n, gl = g;
-    __asm__ volatile("" : "+v"(nl), "+v"(gl));
+    int nl = n;
+    __asm__ volatile("" : "+v"(nl));
     double2 v[LG][4];
 #pragma unroll
     for (int tt = 0; tt < LG; ++tt) {
-      const int c = 16 * (t0 + tt) + nl;
+      if (t0 + tt < NT) {
+        const unsigned cofs = 16u * (unsigned)min(16 * (t0 + tt) + nl, N - 1);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int i = 4 * (j * NT + wave) + gl;
-        const bool ok = t0 + tt < NT && i < N && c < N;
-        v[tt][j] = G[ok ? c + N * i : 0];
-        if (!ok) v[tt][j] = cz(0.0, 0.0);
+        for (int j = 0; j < 4; ++j) v[tt][j] = *reinterpret_cast<const double2 *>(Gb + (cofs + rofs[j]));
       }
     }
 #pragma unroll
     for (int tt = 0; tt < LG; ++tt) {
       const int t = t0 + tt;
       if (t < NT) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {  // the diagonal: + 1/chi0[i], or 1 on the padding
-          const int i = 4 * (j * NT + wave) + g;
-          const double2 d = i < N ? sh.dinv[i] : cz(1.0, 0.0);
-          if (16 * t + n == i) v[tt][j] = cadd(v[tt][j], d);
-        }
+        const bool cok = 16 * t + n < N;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          Vr.set(t, j, (T)v[tt][j].x);
-          Vi.set(t, j, (T)v[tt][j].y);
+          const bool ok = cok && 4 * (j * NT + wave) + g < N;
+          T x = ok ? (T)v[tt][j].x : (T)0, y = ok ? (T)v[tt][j].y : (T)0;
+          if (t == tdiag[j]) {
+            x += dgx[j];
+            y += dgy[j];
+            __asm__ volatile("" : "+v"(x), "+v"(y));  // kept a branch, not a select per tile
+          }
+          Vr.set(t, j, x);
+          Vi.set(t, j, y);
         }
         __asm__ volatile("" : "+v"(Vr.regs(t)), "+v"(Vi.regs(t)));  // converted here, not later
       }
     }
     __asm__ volatile("" ::: "memory");
   }
-  unsigned pmask = 0;  // pivoted columns lane + 64u, bit u (panel layout); the skipped rows' too
-#pragma unroll
-  for (int u = 0; u < NU; ++u)
-    if (lane + 64 * u >= 4 * NPe) pmask |= 1u << u;
   int zero_seen = 0;
 
   // ---- panels: a two-stage software pipeline -----------------------------------
@@ -275,6 +291,14 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
         pr[u][r] = v.x;
         pim[u][r] = v.y;
       }
+    }
+    // the pivoted columns among lane + 64u, bit u (and the columns past NC)
+    unsigned pmask = 0;
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int c = lane + 64 * u;
+      const unsigned bit = c < NC ? (sh.pm[(Ll + 64 * u) >> 5] >> (c & 31)) & 1u : 1u;
+      pmask |= bit << u;
     }
     if constexpr (pre) {
       // panel rows += rows[:, Q] W[wb]: W one column per lane; the rows' values
@@ -398,6 +422,7 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
       const int row = 16 * wave + MT::trow(lane, j1);
       sh.P[bo][lane] = p;
       sh.pcol_of_row[row] = p;
+      atomicOr(&sh.pm[p >> 5], 1u << (p & 31));
     }
     __builtin_amdgcn_s_setprio(0);
   };
@@ -413,11 +438,6 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
     int Ps[4];
 #pragma unroll
     for (int st = 0; st < 4; ++st) Ps[st] = __builtin_amdgcn_readfirstlane(sh.P[bw][st]);
-    if (wave != o) {
-#pragma unroll
-      for (int st = 0; st < 4; ++st)
-        if ((Ps[st] & 63) == lane) pmask |= 1u << (Ps[st] >> 6);
-    }
     if (kp1 < NPe && wave == o1) {
       // the critical path: straight from the barrier to the next barrier; this
       // wave's own MFMA update by W_kp is applied in the next iteration
@@ -477,6 +497,7 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
     for (int jj = 0; jj < 4; ++jj) pcs[jj] = sh.pcol_of_row[16 * wave + MT::trow(g, jj)];
     int it = 0;
     double last_ratio = 0.0;
+    bool converged_stop = false;
     for (; it < max_ir; ++it) {
       // r_i = 1 - sum_j Gamma[j, i] y_j - y_i / chi0_i, stored at column
       // pcol_of_row[i] (where X consumes it); wave w: rows 4k + g for k = w, w + NT, ...
@@ -490,17 +511,21 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
 #pragma unroll
           for (int k = 0; k < NT; ++k) gv[k] = (n + 16 * k < N) ? col[n + 16 * k] : cz(0.0, 0.0);
 #pragma unroll
-          for (int k = 0; k < NT; ++k) acc = cadd(acc, cmul(gv[k], sh.y[n + 16 * k]));
+          for (int k = 0; k < NT; ++k) {  // acc += gv[k] y[n + 16k], as four FMAs
+            const double2 yk = sh.y[n + 16 * k];
+            acc.x = fma(gv[k].x, yk.x, fma(-gv[k].y, yk.y, acc.x));
+            acc.y = fma(gv[k].x, yk.y, fma(gv[k].y, yk.x, acc.y));
+          }
         }
         acc.x = row_sum16(acc.x);
         acc.y = row_sum16(acc.y);
         if (n == 0 && i < N) {
           const double2 yi = sh.y[i];
           const double2 d = cmul(yi, sh.dinv[i]);
-          sh.res[sh.pcol_of_row[pos_of_row(i)]] = cz(1.0 - acc.x - d.x, -acc.y - d.y);
+          sh.res[sh.pcol_of_row[pos_of_row(i)]] = mkc<T>((T)(1.0 - acc.x - d.x), (T)(-acc.y - d.y));
         }
       }
-      if (tid < NC - N) sh.res[N + tid] = cz(0.0, 0.0);
+      if (tid < NC - N) sh.res[N + tid] = mkc<T>(0, 0);
       __syncthreads();
       // dy[p_r] = sum_c X[r][c] r[row_of(c)], r[row_of(c)] stored at c  (FP32 products)
       float dmax = 0.0f, ymax = 0.0f;
@@ -509,10 +534,10 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
         T sx = 0, sy = 0;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-          const double2 rv = sh.res[16 * t + n];
-          const T rx = (T)rv.x, ry = (T)rv.y;
-          sx += Vr.get(t, jj) * rx - Vi.get(t, jj) * ry;
-          sy += Vr.get(t, jj) * ry + Vi.get(t, jj) * rx;
+          const c2 rv = sh.res[16 * t + n];
+          const T rx = rv.x, ry = rv.y;
+          sx = fma(Vr.get(t, jj), rx, fma(-Vi.get(t, jj), ry, sx));
+          sy = fma(Vr.get(t, jj), ry, fma(Vi.get(t, jj), rx, sy));
         }
         sx = row_sum16(sx);
         sy = row_sum16(sy);
@@ -541,14 +566,16 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
       }
       const double ratio = (double)dm / (double)ym;
       // Stop when converged: the correction is at FP64 rounding, or the next
-      // one, extrapolated from the observed contraction ratio / last_ratio, is;
-      // or when stagnating at the FP64 residual's own rounding level
-      // (kappa(A) * 2^-53: the correction no longer halves).
-      const bool converged = !(ratio > 0x1p-50) || (it > 0 && ratio * ratio <= 0x1p-52 * last_ratio);
+      // one, extrapolated from the observed contraction ratio / last_ratio (the
+      // error left in y), is below 2^-48 |y|; or when stagnating at the FP64
+      // residual's own rounding level (kappa(A) * 2^-53: the correction no
+      // longer halves).
+      const bool converged = !(ratio > 0x1p-50) || (it > 0 && ratio * ratio <= 0x1p-48 * last_ratio);
       const bool stagnant = it > 0 && !(ratio < 0.5 * last_ratio);
       const bool stop = converged || stagnant;
       last_ratio = ratio;
       if (stop) {
+        converged_stop = converged;
         ++it;
         break;
       }
@@ -556,10 +583,10 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
     if (tid == 0) {
       atomicMax(&counters[1], (unsigned long long)it);
       atomicAdd(&counters[5], (unsigned long long)it);
-      // a refinement that ended above 2^-30 (or met a zero FP32 pivot) is
-      // listed for the FP64 re-solve when the caller provides the list (room
-      // for every matrix)
-      if (!(last_ratio <= 0x1p-30) || sh.zero_pivot) {
+      // a refinement that did not converge and whose last correction was above
+      // 2^-30 (or that met a zero FP32 pivot) is listed for the FP64 re-solve
+      // when the caller provides the list (room for every matrix)
+      if ((!converged_stop && !(last_ratio <= 0x1p-30)) || sh.zero_pivot) {
         const unsigned long long k = atomicAdd(&counters[2], 1ull);
         if (redo) redo[k] = lb;
       }

@@ -96,6 +96,38 @@ __device__ __forceinline__ float row_sum16<float>(float v) {
   return v;
 }
 
+// FP64: per stage two DPP moves of the partner's halves into fresh registers
+// (every lane of a row reads a lane of the same row, so no old value is kept)
+// and one add; same additions, same order as the generic form
+template <int Q>
+__device__ __forceinline__ double row_stage_f64(double v) {
+  int lo, hi;
+  if constexpr (Q == 0)
+    __asm__("s_nop 1\n\tv_mov_b32_dpp %0, %2 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+            "v_mov_b32_dpp %1, %3 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf"
+            : "=&v"(lo), "=&v"(hi) : "v"(__double2loint(v)), "v"(__double2hiint(v)));
+  else if constexpr (Q == 1)
+    __asm__("s_nop 1\n\tv_mov_b32_dpp %0, %2 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+            "v_mov_b32_dpp %1, %3 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf"
+            : "=&v"(lo), "=&v"(hi) : "v"(__double2loint(v)), "v"(__double2hiint(v)));
+  else if constexpr (Q == 2)
+    __asm__("s_nop 1\n\tv_mov_b32_dpp %0, %2 row_half_mirror row_mask:0xf bank_mask:0xf\n\t"
+            "v_mov_b32_dpp %1, %3 row_half_mirror row_mask:0xf bank_mask:0xf"
+            : "=&v"(lo), "=&v"(hi) : "v"(__double2loint(v)), "v"(__double2hiint(v)));
+  else
+    __asm__("s_nop 1\n\tv_mov_b32_dpp %0, %2 row_mirror row_mask:0xf bank_mask:0xf\n\t"
+            "v_mov_b32_dpp %1, %3 row_mirror row_mask:0xf bank_mask:0xf"
+            : "=&v"(lo), "=&v"(hi) : "v"(__double2loint(v)), "v"(__double2hiint(v)));
+  return v + __hiloint2double(hi, lo);
+}
+template <>
+__device__ __forceinline__ double row_sum16<double>(double v) {
+  v = row_stage_f64<0>(v);
+  v = row_stage_f64<1>(v);
+  v = row_stage_f64<2>(v);
+  return row_stage_f64<3>(v);
+}
+
 // maximum over the 64 lanes of a wave, in lane 63 (unsigned keys)
 __device__ __forceinline__ unsigned wave_max_u32(unsigned k) {
   __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf" : "+v"(k));
