@@ -52,8 +52,11 @@ template <typename T, int NT>
 struct BlkShared {
   typedef typename mfma_traits<T>::c2 c2;
   static constexpr int NC = 16 * NT;
-  c2 W[3][4][NC];      // published panel update (triple-buffered)
-  c2 S[3][16][NC];     // a panel owner's 16 rows, staged (triple-buffered)
+  // rows padded so that rows 1024 bytes apart (NC = 128) do not fall on the
+  // same LDS banks (the column reads of S and the per-group reads of W)
+  static constexpr int SPAD = sizeof(T) == 8 ? 4 : 2, WPAD = sizeof(T) == 8 ? 32 : 16;
+  c2 W[3][4][NC + WPAD];   // published panel update (triple-buffered)
+  c2 S[3][16][NC + SPAD];  // a panel owner's 16 rows, staged (triple-buffered)
   T Cr[NT][16][4];     // per wave: own rows at the four pivot columns (real)
   T Ci[NT][16][4];     // (imaginary)
   int P[3][4];         // pivot columns of the panel
