@@ -73,9 +73,9 @@ typedef struct {
                             (or met a zero FP32 pivot)                       */
   double max_rel_resid;  /* mixed mode: largest final correction |dy|/|y|    */
   double ir_steps_mean;  /* mixed mode: mean refinement step count           */
-  int64_t n_fp64_resolved; /* mixed mode: of n_fallback, matrices re-solved by the
-                            FP64 kernel (all of them when Nnu <= 128; their chi
-                            and gamma are the FP64 results)                  */
+  int64_t n_fp64_resolved; /* mixed mode: of n_fallback, matrices re-solved in
+                            FP64 (all of them; their chi and gamma are the FP64
+                            results)                                         */
 } ldga_bse_report;
 
 /* ---- context --------------------------------------------------------- */

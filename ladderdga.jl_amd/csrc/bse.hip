@@ -273,6 +273,19 @@ int launch_bse_mixed(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, co
 int launch_bse_blk_f64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam,
                        const double2 *chi0, double s, double U, double beta, double *chi, double2 *gamma,
                        unsigned long long *counters, const int *only, int n_only, hipStream_t st);
+int launch_bse_gmem_f64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam,
+                        const double2 *chi0, double s, double U, double beta, double *chi, double2 *gamma,
+                        unsigned long long *cnt, const int *only, int n_only, hipStream_t st);
+
+// FP64 over every matrix or the listed ones: the blocked MFMA kernel up to
+// Nnu = 128, the global-memory kernel above.
+static int launch_fp64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam,
+                       const double2 *chi0, double s, double U, double beta, double *chi, double2 *gamma,
+                       unsigned long long *cnt, const int *only, int n_only, hipStream_t st) {
+  if (N <= 128)
+    return launch_bse_blk_f64(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, chi, gamma, cnt, only, n_only, st);
+  return launch_bse_gmem_f64(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, chi, gamma, cnt, only, n_only, st);
+}
 
 // The register-tile Gauss-Jordan kernel above (Nnu <= 127) is kept as the
 // second FP64 implementation; LDGA_BSE_F64=gj selects it instead of the
@@ -292,15 +305,14 @@ int launch_bse(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double 
   unsigned long long *cnt = (unsigned long long *)ldga_buf(ctx, "bse_counters", 16 * 8);
   if (!cnt) return LDGA_ERR_HIP;
   LDGA_HIP_TRY(ctx, hipMemsetAsync(cnt, 0, 16 * 8, st));
-  // mixed mode with an FP64 kernel for this size: the refinement's failures
-  // are listed (room for every matrix) and re-solved in FP64
-  const bool resolve = precision == LDGA_PREC_MIXED && N <= 128;
+  // mixed mode: the refinement's failures are listed (room for every matrix)
+  // and re-solved in FP64
+  const bool resolve = precision == LDGA_PREC_MIXED;
   int *redo = nullptr;
   if (resolve && !(redo = (int *)ldga_buf(ctx, "bse_redo", (size_t)Nq * Nw * sizeof(int)))) return LDGA_ERR_HIP;
   int rc;
   if (precision == LDGA_PREC_FP64 && !use_register_gj()) {
-    rc = launch_bse_blk_f64(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, nullptr, 0,
-                            st);
+    rc = launch_fp64(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, nullptr, 0, st);
   } else if (precision == LDGA_PREC_FP64) {
     if (N <= 47)
       rc = run_gj<3, 3, 16>(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, st);
@@ -311,7 +323,7 @@ int launch_bse(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double 
     else if (N <= 127)
       rc = run_gj<8, 4, 32>(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, st);
     else
-      return ldga_fail(ctx, LDGA_ERR_SHAPE, "FP64 BSE supports Nnu <= 127 (got %d)", N);
+      return ldga_fail(ctx, LDGA_ERR_SHAPE, "the register Gauss-Jordan kernel (LDGA_BSE_F64=gj) supports Nnu <= 127 (got %d)", N);
   } else if (precision == LDGA_PREC_MIXED) {
     rc = launch_bse_mixed(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, redo, st);
   } else {
@@ -325,8 +337,8 @@ int launch_bse(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double 
   }
   if (resolve && h[2]) {
     // the listed matrices' chi and gamma are overwritten by the FP64 kernel
-    rc = launch_bse_blk_f64(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt + 8, redo,
-                            (int)h[2], st);
+    rc = launch_fp64(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt + 8, redo, (int)h[2],
+                     st);
     if (rc) return rc;
     LDGA_HIP_TRY(ctx, hipMemcpyAsync(h2, cnt + 8, sizeof h2, hipMemcpyDeviceToHost, st));
     LDGA_HIP_TRY(ctx, hipStreamSynchronize(st));

@@ -347,12 +347,12 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
       // argmax over the unpivoted columns of panel row st, as one 32-bit key per
       // lane: the float bits of |re| + |im| (monotone for non-negative values)
       // with the low byte replaced by 255 - column, so ties go to the smaller
-      // column; pivoted and padding columns have key 0.
+      // column; pivoted and padding columns have key 0.  A NaN value has the
+      // largest key: its column is picked and the NaN propagates.
       unsigned key = 0;
 #pragma unroll
       for (int u = 0; u < NU; ++u) {
-        float v = (float)(fabs(pr[u][st]) + fabs(pim[u][st]));
-        if (!(v >= 0.0f)) v = 0.0f;  // NaN input: still pick a column, NaN propagates
+        const float v = (float)(fabs(pr[u][st]) + fabs(pim[u][st]));
         const unsigned k = (__float_as_uint(v) & 0xffffff00u) | (unsigned)(255 - (lane + 64 * u));
         if (!((pmask >> u) & 1u)) key = max(key, k);
       }

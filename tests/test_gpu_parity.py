@@ -107,7 +107,8 @@ def test_bse_register_gj_kernel(ctx, monkeypatch):
 
 
 def test_bse_mixed_nnu160(ctx):
-    """Nν = 160 (the 32^3 workload's matrix size) on a small 3D grid, mixed precision vs LAPACK."""
+    """Nν = 160 (the 32^3 workload's matrix size) on a small 3D grid: mixed precision (FP32 MFMA inverse)
+    and FP64 (the global-memory Gauss-Jordan kernel used above Nν = 128) vs LAPACK."""
     import ladderdga_hip as L
     inp, oK, omP, osP = setup("tiny", D=3, Ns=4, n_iν=80, n_iω=1, U=2.0, β=25.0, n=1.0, t=0.408248290463863)
     print("bse mixed Nν=160", flush=True)
@@ -119,8 +120,25 @@ def test_bse_mixed_nnu160(ctx):
         rχ, rγ = O.calc_χγ(ch, Γ, ref_χ0, oK, omP, osP)
         assert rep["n_fallback"] == 0
         assert relerr(χ.data, rχ) <= TOL and relerr(γ.data, rγ) <= TOL
-    with pytest.raises(L.LdgaError):
-        L.calc_χγ("m", inp.Γ_m, χ0, inp.kG, inp.mP, inp.sP, precision="fp64", ctx=ctx)
+        χ, γ = L.calc_χγ(ch, Γ, χ0, inp.kG, inp.mP, inp.sP, precision="fp64", ctx=ctx)
+        assert relerr(χ.data, rχ) <= TOL and relerr(γ.data, rγ) <= TOL
+
+
+@pytest.mark.parametrize("n_iν", [65, 100])
+def test_bse_fp64_global_memory_kernel(ctx, n_iν):
+    """FP64 above Nν = 128 (Nν = 130, 200: no tile or panel alignment) runs the global-memory
+    Gauss-Jordan kernel; both channels against LAPACK."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP = setup("tiny", D=3, Ns=4, n_iν=n_iν, n_iω=1, U=1.5, β=12.0, n=1.0, t=0.408248290463863)
+    print(f"bse fp64 global memory Nν={2 * n_iν}", flush=True)
+    ref_χ0 = O.calc_bubble(inp.G_fft, inp.G_rfft, oK, omP, osP)
+    χ0 = L.χ0T(np.asfortranarray(ref_χ0), inp.mP.β)
+    for ch, Γ in (("m", inp.Γ_m), ("d", inp.Γ_d)):
+        rep = {}
+        χ, γ = L.calc_χγ(ch, Γ, χ0, inp.kG, inp.mP, inp.sP, precision="fp64", ctx=ctx, report=rep)
+        rχ, rγ = O.calc_χγ(ch, Γ, ref_χ0, oK, omP, osP)
+        assert rep["n_zero_pivot"] == 0
+        assert relerr(χ.data, rχ) <= TOL and relerr(γ.data, rγ) <= TOL
 
 
 @pytest.mark.parametrize("n_iν", [1, 2, 13, 28, 36, 44, 52, 59, 60, 68])
@@ -164,10 +182,10 @@ RESOLVE_CASES = [(dict(D=2, Ns=4, n_iν=4, n_iω=3), [(0, 0)]),
 
 @pytest.mark.parametrize("kw,targets", RESOLVE_CASES)
 def test_bse_mixed_refinement_failure_resolved_in_fp64(ctx, kw, targets):
-    """Matrices of condition number 1e9 are beyond an FP32 inverse: their refinement fails.  Up to
-    Nν = 128 the library re-solves exactly those with the FP64 kernel, so their χ, γ equal the FP64
-    call's bit for bit; at Nν = 160 (no FP64 kernel) they are reported and the call warns.  Every other
-    (q, ω) meets the FP64 bar against LAPACK either way."""
+    """Matrices of condition number 1e9 are beyond an FP32 inverse: their refinement fails.  The library
+    re-solves exactly those with the FP64 kernel of that size (blocked MFMA up to Nν = 128, global memory
+    above), so their χ, γ equal the FP64 call's bit for bit.  Every other (q, ω) meets the FP64 bar
+    against LAPACK."""
     import warnings
     import ladderdga_hip as L
     inp, oK, omP, osP = setup("tiny", **kw)
@@ -186,15 +204,12 @@ def test_bse_mixed_refinement_failure_resolved_in_fp64(ctx, kw, targets):
         χ, γ = L.calc_χγ("m", Γ, χ0, inp.kG, inp.mP, inp.sP, precision="mixed", ctx=ctx, report=rep)
     rχ, rγ = O.calc_χγ("m", Γ, ref_χ0, oK, omP, osP)
     assert rep["n_fallback"] == len(bad) and rep["n_zero_pivot"] == 0
-    if Nν <= 128:
-        assert rep["n_fp64_resolved"] == rep["n_fallback"] and not warned
-        χf, γf = L.calc_χγ("m", Γ, χ0, inp.kG, inp.mP, inp.sP, precision="fp64", ctx=ctx)
-        for q, w in bad:
-            assert χ.data[q, w] == χf.data[q, w] and np.array_equal(γ.data[q, :, w], γf.data[q, :, w])
-            # FP64 against LAPACK at κ = 1e9: both are off the exact solution by up to ~κ 2^-53 Nν
-            assert relerr(χ.data[q, w], rχ[q, w]) <= 1e-4 and relerr(γ.data[q, :, w], rγ[q, :, w]) <= 1e-4
-    else:
-        assert rep["n_fp64_resolved"] == 0 and len(warned) == 1
+    assert rep["n_fp64_resolved"] == rep["n_fallback"] and not warned
+    χf, γf = L.calc_χγ("m", Γ, χ0, inp.kG, inp.mP, inp.sP, precision="fp64", ctx=ctx)
+    for q, w in bad:
+        assert χ.data[q, w] == χf.data[q, w] and np.array_equal(γ.data[q, :, w], γf.data[q, :, w])
+        # FP64 against LAPACK at κ = 1e9: both are off the exact solution by up to ~κ 2^-53 Nν
+        assert relerr(χ.data[q, w], rχ[q, w]) <= 1e-4 and relerr(γ.data[q, :, w], rγ[q, :, w]) <= 1e-4
     keep = np.ones(rχ.shape, dtype=bool)
     for q, w in bad:
         keep[q, w] = False
