@@ -217,13 +217,22 @@ def test_bse_mixed_refinement_failure_resolved_in_fp64(ctx, kw, targets):
     assert relerr(γ.data.transpose(0, 2, 1)[keep], rγ.transpose(0, 2, 1)[keep]) <= TOL
 
 
-def test_bse_zero_pivot_reports_and_gives_nan(ctx):
-    """A = Γ + diag(1/χ₀) exactly singular -> LDGA_ERR_ZERO_PIVOT status and NaN outputs."""
+ZERO_PIVOT_CASES = [("tiny", TINY[0], 0), ("tiny", TINY[0], 1),
+                    ("tiny", dict(D=3, Ns=4, n_iν=65, n_iω=1, U=1.5, β=12.0, n=1.0, t=0.408248290463863), 0),
+                    ("tiny", dict(D=3, Ns=4, n_iν=65, n_iω=1, U=1.5, β=12.0, n=1.0, t=0.408248290463863), 1)]
+
+
+@pytest.mark.parametrize("name,kw,precision", ZERO_PIVOT_CASES)
+def test_bse_zero_pivot_reports_and_gives_nan(ctx, name, kw, precision):
+    """A = Γ + diag(1/χ₀) exactly singular -> LDGA_ERR_ZERO_PIVOT status and NaN outputs: FP64 in the blocked
+    kernel (Nν = 8) and the global-memory kernel (Nν = 130), and mixed, whose zero-pivot matrices are
+    re-solved in FP64."""
     import ctypes as C
     import ladderdga_hip as L
     from ladderdga_hip import _lib
     from ladderdga_hip.api import _freq
-    inp, oK, omP, osP = setup("tiny", **TINY[0])
+    inp, oK, omP, osP = setup(name, **kw)
+    print(f"bse zero pivot Nν={2 * inp.sP.n_iν} precision={precision}", flush=True)
     χ0 = O.calc_bubble(inp.G_fft, inp.G_rfft, oK, omP, osP)
     Nν, Nω = 2 * inp.sP.n_iν, 2 * inp.sP.n_iω + 1
     Γ = np.zeros((Nν, Nν, Nω), dtype=complex, order="F")
@@ -234,10 +243,12 @@ def test_bse_zero_pivot_reports_and_gives_nan(ctx):
     rep = _lib.BseReport()
     ctx.set_kgrid(inp.kG)
     χ0f = np.asfortranarray(χ0)  # keep the Fortran copy alive across the call
-    rc = _lib.lib().ldga_chi_gamma(ctx.handle, 1, 0, C.byref(_freq(inp.sP)), inp.mP.U, inp.mP.β,
+    rc = _lib.lib().ldga_chi_gamma(ctx.handle, 1, precision, C.byref(_freq(inp.sP)), inp.mP.U, inp.mP.β,
                                    Γ.ctypes.data, χ0f.ctypes.data, χ.ctypes.data, γ.ctypes.data, C.byref(rep))
     assert rc == _lib.LDGA_ERR_ZERO_PIVOT
     assert rep.n_zero_pivot >= 1
+    if precision == 1:
+        assert rep.n_fp64_resolved == rep.n_fallback >= 1
     # A(q, ω=0) = diag(1/χ0[q] - 1/χ0[0]): singular where a diagonal entry vanishes.  Two different χ0
     # values can have the same rounded reciprocal, so only clearly nonzero diagonals must give finite χ.
     dmin = np.min(np.abs(1.0 / χ0[:, :, 0] - 1.0 / χ0[0:1, :, 0]), axis=1)
@@ -247,7 +258,8 @@ def test_bse_zero_pivot_reports_and_gives_nan(ctx):
     assert np.all(np.isfinite(χ[safe, 0])) and np.all(np.isfinite(χ[:, 1:]))
     assert safe.sum() >= 1
     with pytest.warns(UserWarning):
-        L.calc_χγ("m", Γ, L.χ0T(np.asfortranarray(χ0), inp.mP.β), inp.kG, inp.mP, inp.sP, ctx=ctx)
+        L.calc_χγ("m", Γ, L.χ0T(np.asfortranarray(χ0), inp.mP.β), inp.kG, inp.mP, inp.sP, ctx=ctx,
+                  precision=["fp64", "mixed"][precision])
 
 
 @pytest.mark.parametrize("name,kw", CASES[:3] + [("2d16", {})])
