@@ -287,6 +287,15 @@ static int launch_fp64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, 
   return launch_bse_gmem_f64(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, chi, gamma, cnt, only, n_only, st);
 }
 
+// Clears the report counters with a kernel rather than hipMemsetAsync, so that
+// a caller may capture the FP64 path (no host synchronisation without a
+// report) in a HIP graph.
+namespace {
+__global__ void bse_clear_counters(unsigned long long *cnt, int n) {
+  if ((int)threadIdx.x < n) cnt[threadIdx.x] = 0ull;
+}
+}  // namespace
+
 // The register-tile Gauss-Jordan kernel above (Nnu <= 127) is kept as the
 // second FP64 implementation; LDGA_BSE_F64=gj selects it instead of the
 // blocked MFMA kernel (bse_blk.hip).
@@ -304,7 +313,8 @@ int launch_bse(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double 
   // matrices; [8..15]: the same for the FP64 re-solve
   unsigned long long *cnt = (unsigned long long *)ldga_buf(ctx, "bse_counters", 16 * 8);
   if (!cnt) return LDGA_ERR_HIP;
-  LDGA_HIP_TRY(ctx, hipMemsetAsync(cnt, 0, 16 * 8, st));
+  bse_clear_counters<<<1, 16, 0, st>>>(cnt, 16);
+  LDGA_HIP_TRY(ctx, hipGetLastError());
   // mixed mode: the refinement's failures are listed (room for every matrix)
   // and re-solved in FP64
   const bool resolve = precision == LDGA_PREC_MIXED;

@@ -185,17 +185,19 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
 
   // ---- panels: a two-stage software pipeline -----------------------------------
   // Panel kp = element kp / NT of the lane groups of wave kp % NT.  Its owner
-  // stages its raw panel rows in LDS at the end of iteration kp-2 (after the
-  // MFMA update by W_{kp-2}) and factors the panel one iteration early: in
-  // iteration kp-1 it applies W_{kp-1} to the staged rows one column per lane
-  // (the pivot-column values come from the same copy), factors them and
-  // publishes W_kp and goes straight to the barrier; the MFMA update of its
-  // whole tile by W_{kp-1} (A operand gathered then) follows in iteration kp.  The serial chain per iteration is thus:
-  // barrier, rank-4 update of four staged rows, four pivot steps, publish; the
-  // other waves' MFMA updates overlap it.  W and P are triple-buffered (W_kp is
-  // read up to iteration kp+1 and rewritten by the owner of panel kp+3 after
-  // the barrier of iteration kp+2); the staging buffer is double-buffered and
-  // written and read by the same wave.
+  // stages all 16 of its rows in LDS (S[kp % 3]) at the end of iteration kp-2,
+  // after its MFMA update by W_{kp-2}, and factors the panel one iteration
+  // early: in iteration kp-1 it loads the panel rows from S one column per
+  // lane, applies W_{kp-1} to them in registers (the rows' values at the pivot
+  // columns by readlane), factors them, publishes W_kp and goes straight to
+  // the barrier.  Its own tile misses the update by W_{kp-1}: in iteration kp
+  // it applies W_{kp-1} and W_kp together, both A operands taken from S, and
+  // then writes the panel's final rows.  The serial chain per iteration is
+  // thus: barrier, rank-4 update of four staged rows, four pivot steps,
+  // publish; the other waves' MFMA updates overlap it.  W, P and S are
+  // triple-buffered: W_kp is read up to iteration kp+1 and rewritten by the
+  // owner of panel kp+3 after the barrier of iteration kp+2; S[kp % 3] is read
+  // up to iteration kp and rewritten at the end of iteration kp+1.
   static_assert(NT >= 2, "the pipeline needs two waves");
   // Per-lane parts of the LDS addresses, re-laundered every iteration: the
   // compiler then forms each address as base + immediate offset instead of
