@@ -510,11 +510,19 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
         const int i = ib + g;
         double2 acc = cz(0.0, 0.0);
         if (i < N) {
-          // all NT loads of the row in flight at once (L2-resident Gamma)
+          // all NT loads of the row in flight at once (L2-resident Gamma); when
+          // every tile but the last lies inside the matrix (N > 16 (NT - 1), a
+          // wave-uniform test) only the last tile's load is guarded
           const double2 *col = G + (size_t)N * i;
           double2 gv[NT];
+          if (N > 16 * (NT - 1)) {
 #pragma unroll
-          for (int k = 0; k < NT; ++k) gv[k] = (n + 16 * k < N) ? col[n + 16 * k] : cz(0.0, 0.0);
+            for (int k = 0; k < NT - 1; ++k) gv[k] = col[n + 16 * k];
+            gv[NT - 1] = (n + 16 * (NT - 1) < N) ? col[n + 16 * (NT - 1)] : cz(0.0, 0.0);
+          } else {
+#pragma unroll
+            for (int k = 0; k < NT; ++k) gv[k] = (n + 16 * k < N) ? col[n + 16 * k] : cz(0.0, 0.0);
+          }
 #pragma unroll
           for (int k = 0; k < NT; ++k) {  // acc += gv[k] y[n + 16k], as four FMAs
             const double2 yk = sh.y[n + 16 * k];
