@@ -275,6 +275,23 @@ def main():
         "mixed_rel_err_vs_fp64": errs,
     }
     result["lambdadm_newton_step"] = lambdadm_step_costs(L, ctx, out[main_prec], λ0, inp, numax, Nq, Nν, Nω)
+    # G tables (helpers.jl:153-183) from the wedge G_loc, as the self-consistency loop rebuilds them:
+    # algorithmic bytes = wedge G read + both full tables written
+    gw = up(inp.gLoc)
+    ga, gb = torch.empty_like(Gf), torch.empty_like(Gr)
+    ts = []
+    for it in range(4):
+        e0, e1 = ev(), ev()
+        e0.record(stream)
+        ctx.check(Lb.ldga_g_tables_dev(ctx.handle, frp, gw.data_ptr(), ga.data_ptr(), gb.data_ptr(), sp))
+        e1.record(stream)
+        torch.cuda.synchronize()
+        if it:
+            ts.append(e0.elapsed_time(e1))
+    gt_ms = float(np.mean(ts))
+    gt_bytes = Nq * Nfft * 16 + 2 * Nk * Nfft * 16
+    result["g_tables"] = {"ms": gt_ms, "GBps": gt_bytes / (gt_ms * 1e-3) / 1e9,
+                          "max_rel_err_vs_input_tables": max(rel(ga, Gf), rel(gb, Gr))}
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(inp, numax, args.cpu_points)
     ctx.close()

@@ -114,6 +114,15 @@ int ldga_bubble(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double *G
 int ldga_bubble_dev(ldga_ctx *ctx, const ldga_freq *fr, double beta, const void *G_fft, const void *G_rfft,
                     void *chi0, void *stream);
 
+/* ---- G tables: G_fft / G_rfft (src/helpers.jl:153-183) ------------------
+ * From G on the wedge over fft_range (Nq x n_fft, column-major, e.g. the
+ * G_ladder of ldga_lambdadm_residual), the full-grid tables
+ * G_fft = fft(expandKArr(G)) and G_rfft = fft(reverse(expandKArr(G)))
+ * (Ns^D x n_fft each), the inputs of ldga_bubble and ldga_sigma. */
+int ldga_g_tables(ldga_ctx *ctx, const ldga_freq *fr, const double *G_wedge, double *G_fft, double *G_rfft);
+int ldga_g_tables_dev(ldga_ctx *ctx, const ldga_freq *fr, const void *G_wedge, void *G_fft, void *G_rfft,
+                      void *stream);
+
 /* ---- Bethe-Salpeter: calc_chi_gamma (src/BSETools.jl:192-248) -----------
  * Replaces the per-(q,w) LapackWrapper.inv! (src/LapackWrapper.jl:64-72,
  * zgetrf_+zgetri_) and the plain sums: chi = Re sum(A^-1)/beta^2,

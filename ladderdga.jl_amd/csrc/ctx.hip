@@ -330,6 +330,32 @@ int ldga_bubble_dev(ldga_ctx *ctx, const ldga_freq *fr, double beta, const void 
                        ldga_stream(ctx, stream));
 }
 
+int ldga_g_tables(ldga_ctx *ctx, const ldga_freq *fr, const double *G_wedge, double *G_fft, double *G_rfft) {
+  if (!ctx || !G_wedge || !G_fft || !G_rfft) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  NEED_KGRID(ctx);
+  int rc = ldga_check_freq(ctx, fr);
+  if (rc) return rc;
+  hipSetDevice(ctx->device);
+  const size_t wbytes = (size_t)ctx->Nq * fr->n_fft * 16, gbytes = (size_t)ctx->oct.Nk * fr->n_fft * 16;
+  void *w, *a, *b;
+  if ((rc = h2d(ctx, "G_wedge", G_wedge, wbytes, &w))) return rc;
+  if (!(a = ldga_buf(ctx, "G_fft", gbytes)) || !(b = ldga_buf(ctx, "G_rfft", gbytes))) return LDGA_ERR_HIP;
+  if ((rc = launch_gtables(ctx, fr, (const double2 *)w, (double2 *)a, (double2 *)b, ctx->stream))) return rc;
+  if ((rc = d2h(ctx, G_fft, a, gbytes))) return rc;
+  return d2h(ctx, G_rfft, b, gbytes);
+}
+
+int ldga_g_tables_dev(ldga_ctx *ctx, const ldga_freq *fr, const void *G_wedge, void *G_fft, void *G_rfft,
+                      void *stream) {
+  if (!ctx || !G_wedge || !G_fft || !G_rfft) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  NEED_KGRID(ctx);
+  int rc = ldga_check_freq(ctx, fr);
+  if (rc) return rc;
+  hipSetDevice(ctx->device);
+  return launch_gtables(ctx, fr, (const double2 *)G_wedge, (double2 *)G_fft, (double2 *)G_rfft,
+                        ldga_stream(ctx, stream));
+}
+
 int ldga_chi_gamma(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double U, double beta,
                    const double *Gamma, const double *chi0, double *chi, double *gamma, ldga_bse_report *report) {
   if (!ctx || !Gamma || !chi0 || !chi || !gamma) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");

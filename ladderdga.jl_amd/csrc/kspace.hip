@@ -402,6 +402,46 @@ __global__ void sum_pairs_kernel(int n, const double2 *__restrict__ partial, dou
   }
 }
 
+// ---------------------------------------------------------------------------
+// G tables (src/helpers.jl:153-183): G_fft = fft(expand(G)) and
+// G_rfft = fft(reverse(expand(G))) for every nu of fft_range, from G on the
+// wedge.  G is even in every axis, so fft(expand(G)) is the octant cosine
+// transform of the weighted octant values, and is itself even in R.
+// reverse() shifts every axis by one point (reverse(G)[i] = G[i + 1] for an
+// even G), so G_rfft(R) = exp(2 pi i sum_d R_d / Ns) G_fft(R).  One workgroup
+// per nu: the octant in LDS, D passes, then both full grids written.
+// ---------------------------------------------------------------------------
+template <int MAXPT>
+__global__ __launch_bounds__(kNT) void gtables_kernel(OctTables T, const double2 *__restrict__ Gw,
+                                                      double2 *__restrict__ Gfft, double2 *__restrict__ Grfft) {
+  extern __shared__ double2 smem[];
+  double2 *buf = smem;
+  double *cs = reinterpret_cast<double *>(smem + T.NO);
+  const int f = blockIdx.x;  // position in fft_range
+  const double2 *g = Gw + (size_t)T.Nq * f;
+  for (int m = threadIdx.x; m < T.Ns; m += kNT) cs[m] = T.cosT[m];
+  for (int o = threadIdx.x; o < T.NO; o += kNT) buf[o] = cscale(g[T.oct2wedge[o]], T.octw[o]);
+  __syncthreads();
+  for (int ax = 0; ax < T.D; ++ax) oct_pass<MAXPT>(buf, cs, ax, T);
+  // the full grid, first axis fastest (column-major like the reference's arrays)
+  double2 *a = Gfft + (size_t)T.Nk * f, *b = Grfft + (size_t)T.Nk * f;
+  const int Nk = (int)T.Nk;
+  for (int i = threadIdx.x; i < Nk; i += kNT) {
+    int rem = i, o = 0, sum = 0;
+    for (int d = 0; d < T.D; ++d) {
+      const int r = rem % T.Ns;
+      rem /= T.Ns;
+      o += min(r, T.Ns - r) * T.stride[d];
+      sum += r;
+    }
+    const double2 v = buf[o];
+    a[i] = v;
+    double sn, cn;
+    sincospi(2.0 * (double)(sum % T.Ns) / (double)T.Ns, &sn, &cn);
+    b[i] = cmul(v, cz(cn, sn));
+  }
+}
+
 }  // namespace
 
 // number of register slots per thread needed for the octant (NO / 256 rounded up)
@@ -451,6 +491,14 @@ int launch_bubble(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double2
   const double scale = -beta / (T.Nk * T.Nk);
   OCT_DISPATCH(T.NO, smem, bubble_kernel, Nnu * Nw, st, T, Nnu, ntot, fr->n_iw, fr->shift, fr->fft_first, Gfft,
                Grfft, chi0, scale);
+  return LDGA_OK;
+}
+
+int launch_gtables(ldga_ctx *ctx, const ldga_freq *fr, const double2 *Gw, double2 *Gfft, double2 *Grfft,
+                   hipStream_t st) {
+  const OctTables &T = ctx->oct;
+  const size_t smem = (size_t)T.NO * 16 + (size_t)T.Ns * 8;
+  OCT_DISPATCH(T.NO, smem, gtables_kernel, fr->n_fft, st, T, Gw, Gfft, Grfft);
   return LDGA_OK;
 }
 

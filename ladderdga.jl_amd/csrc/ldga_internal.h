@@ -92,6 +92,8 @@ hipStream_t ldga_stream(ldga_ctx *ctx, void *stream);
 int ldga_check_freq(ldga_ctx *ctx, const ldga_freq *fr);
 
 // kernel launchers (device pointers, asynchronous) ---------------------------
+int launch_gtables(ldga_ctx *ctx, const ldga_freq *fr, const double2 *Gw, double2 *Gfft, double2 *Grfft,
+                   hipStream_t st);
 int launch_bubble(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double2 *Gfft, const double2 *Grfft,
                   double2 *chi0, hipStream_t st);
 int launch_bse(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double U, double beta, const double2 *Gam,

@@ -5,7 +5,7 @@ Python mirror of the reference API over the C ABI of libldga_hip.so.
 from ._lib import Context, LdgaError, default_context, lib  # noqa: F401
 from .kgrid import KGrid, gen_kGrid  # noqa: F401
 from .types import ModelParameters, SimulationParameters, χ0T, χT, γT  # noqa: F401
-from .api import (calc_bubble, calc_χγ, calc_λ0, calc_Σ, calc_Σ_parts, χ_λ, dχ_λ, χ_λ_inplace, reset,  # noqa: F401
+from .api import (G_fft, calc_bubble, calc_χγ, calc_λ0, calc_Σ, calc_Σ_parts, χ_λ, dχ_λ, χ_λ_inplace, reset,  # noqa: F401
                   sum_kω, get_λ_min)
 
 from .lambda_correction import (λ_result, newton_right, newton_right_1d, λm_correction, λdm_correction,  # noqa: F401
@@ -13,5 +13,5 @@ from .lambda_correction import (λ_result, newton_right, newton_right_1d, λm_co
 
 __all__ = ["λ_result", "newton_right", "newton_right_1d", "λm_correction", "λdm_correction", "LambdaDmProblem",
            "gen_νω_indices", "Context", "LdgaError", "KGrid", "gen_kGrid", "ModelParameters", "SimulationParameters", "χ0T", "χT",
-           "γT", "calc_bubble", "calc_χγ", "calc_λ0", "calc_Σ", "calc_Σ_parts", "χ_λ", "dχ_λ", "χ_λ_inplace",
+           "γT", "G_fft", "calc_bubble", "calc_χγ", "calc_λ0", "calc_Σ", "calc_Σ_parts", "χ_λ", "dχ_λ", "χ_λ_inplace",
            "reset", "sum_kω", "get_λ_min"]

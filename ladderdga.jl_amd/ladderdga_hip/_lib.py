@@ -29,7 +29,7 @@ EOM_KINDS = {"full": 0, "χ_m": 1, "γ_m": 2, "χ_d": 3, "γ_d": 4, "rest_01": 5
 EXPORTED = [
     "ldga_version", "ldga_create", "ldga_destroy", "ldga_last_error", "ldga_sync",
     "ldga_set_kgrid", "ldga_default_wedge",
-    "ldga_bubble", "ldga_bubble_dev",
+    "ldga_g_tables", "ldga_g_tables_dev", "ldga_bubble", "ldga_bubble_dev",
     "ldga_chi_gamma", "ldga_chi_gamma_dev",
     "ldga_lambda0", "ldga_lambda0_dev",
     "ldga_sum_kw", "ldga_sum_kw_dev",
@@ -87,6 +87,8 @@ def lib():
             "ldga_sync": (i32, [vp]),
             "ldga_set_kgrid": (i32, [vp, i32, i32, i32, vp, vp]),
             "ldga_default_wedge": (i32, [i32, i32, vp, vp]),
+            "ldga_g_tables": (i32, [vp, C.POINTER(Freq), vp, vp, vp]),
+            "ldga_g_tables_dev": (i32, [vp, C.POINTER(Freq), vp, vp, vp, vp]),
             "ldga_bubble": (i32, [vp, C.POINTER(Freq), dbl, vp, vp, vp]),
             "ldga_bubble_dev": (i32, [vp, C.POINTER(Freq), dbl, vp, vp, vp, vp]),
             "ldga_chi_gamma": (i32, [vp, i32, i32, C.POINTER(Freq), dbl, dbl, vp, vp, vp, vp, C.POINTER(BseReport)]),

@@ -35,6 +35,25 @@ def _ctx(ctx, kG):
 
 
 # ---------------------------------------------------------------------------
+# G tables (src/helpers.jl:153-183)
+# ---------------------------------------------------------------------------
+def G_fft(G, kG: KGrid, sP: SimulationParameters, ctx=None):
+    """(G_fft, G_rfft) = (fft(expandKArr(G)), fft(reverse(expandKArr(G)))) for every ν of
+    sP.fft_range, from G on the wedge (Nq x length(fft_range)), as the reference's
+    G_fft(G, kG, sP); the result feeds calc_bubble and calc_Σ."""
+    ctx = _ctx(ctx, kG)
+    nf = len(sP.fft_range)
+    if G.shape != (kG.Nq, nf):
+        raise ValueError(f"G must be Nq x length(fft_range) = {kG.Nq} x {nf}")
+    shape = kG.gridshape() + (nf,)
+    out = np.empty(shape, dtype=c128, order="F")
+    out_r = np.empty(shape, dtype=c128, order="F")
+    ctx.check(_lib.lib().ldga_g_tables(ctx.handle, C.byref(_freq(sP)), ptr(G, c128, "G"), out.ctypes.data,
+                                       out_r.ctypes.data))
+    return out, out_r
+
+
+# ---------------------------------------------------------------------------
 # bubble (src/BSETools.jl:157-172)
 # ---------------------------------------------------------------------------
 def calc_bubble(Gνω, Gνω_r, kG: KGrid, mP: ModelParameters, sP: SimulationParameters, local_tail=False, ctx=None):

@@ -261,6 +261,24 @@ def G_tables(kG: KGrid, mP: ModelParameters, sP: SimulationParameters, Σ_loc):
     return gLoc, g_fft, g_rfft
 
 
+def G_fft_from_wedge(G_wedge, kG: KGrid):
+    """G_fft!/G_rfft! (src/helpers.jl:165-183): for every column ν of G_wedge
+    (Nq x n), expandKArr to the full grid, then fft, and fft of reverse.
+
+    Returns (G_fft, G_rfft), each gridshape x n.
+    """
+    G_wedge = np.asarray(G_wedge)
+    shape = (kG.Ns,) * kG.D
+    n = G_wedge.shape[1]
+    g_fft = np.empty(shape + (n,), dtype=complex)
+    g_rfft = np.empty(shape + (n,), dtype=complex)
+    for i in range(n):
+        g = kG.expand(G_wedge[:, i])
+        g_fft[..., i] = np.fft.fftn(g)
+        g_rfft[..., i] = np.fft.fftn(g[(slice(None, None, -1),) * kG.D])   # Julia reverse(A)
+    return g_fft, g_rfft
+
+
 # ----------------------------------------------------------------------------
 # bubble (src/BSETools.jl:157-172)
 # ----------------------------------------------------------------------------

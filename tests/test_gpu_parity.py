@@ -47,6 +47,35 @@ CASES = [case(i) for i in range(len(TINY))] + [("2d16", {})]
 
 
 @pytest.mark.parametrize("name,kw", CASES)
+def test_g_tables(ctx, name, kw):
+    """G_fft/G_rfft (helpers.jl:153-183) on the GPU from a wedge G: the local G of the inputs gives the
+    tables the inputs were built with on the full grid; a random k-dependent G matches the oracle."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP = setup(name, **kw)
+    print(f"g tables {name} {kw}", flush=True)
+    f, r = L.G_fft(np.asfortranarray(inp.gLoc), inp.kG, inp.sP, ctx=ctx)
+    assert relerr(f, inp.G_fft) <= 1e-12 and relerr(r, inp.G_rfft) <= 1e-12
+    rng = np.random.default_rng(5)
+    Gw = np.asfortranarray(rng.standard_normal(inp.gLoc.shape) + 1j * rng.standard_normal(inp.gLoc.shape))
+    f, r = L.G_fft(Gw, inp.kG, inp.sP, ctx=ctx)
+    rf, rr = O.G_fft_from_wedge(Gw, oK)
+    assert relerr(f, rf) <= 1e-12 and relerr(r, rr) <= 1e-12
+    # the tables feed the bubble as the reference's do
+    χ0 = L.calc_bubble(f, r, inp.kG, inp.mP, inp.sP, ctx=ctx)
+    assert relerr(χ0.data, O.calc_bubble(rf, rr, oK, omP, osP)) <= TOL
+
+
+@pytest.mark.parametrize("name", ["2d64", "3d24"])
+def test_g_tables_large(ctx, name):
+    """Full-size G tables from the wedge G_loc against the full-grid numpy tables."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP = setup(name)
+    print(f"g tables large {name}", flush=True)
+    f, r = L.G_fft(np.asfortranarray(inp.gLoc), inp.kG, inp.sP, ctx=ctx)
+    assert relerr(f, inp.G_fft) <= 1e-12 and relerr(r, inp.G_rfft) <= 1e-12
+
+
+@pytest.mark.parametrize("name,kw", CASES)
 def test_bubble(ctx, name, kw):
     import ladderdga_hip as L
     inp, oK, omP, osP = setup(name, **kw)

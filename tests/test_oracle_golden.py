@@ -192,6 +192,27 @@ def _tiny_inputs(D=2, Ns=4, n_inu=3, n_iw=2, U=1.0, β=4.0, seed=11):
     return kG, sP_, mP, Σ_loc, gLoc, gf, gr, np.random.default_rng(seed)
 
 
+@pytest.mark.parametrize("D,Ns", [(2, 4), (2, 6), (3, 4)])
+def test_G_fft_from_wedge(D, Ns):
+    """G_fft!/G_rfft! from a wedge G: equals the tables built on the full grid (local Σ), a
+    brute-force DFT of expand(G) and of its reverse for a k-dependent G, and the shift identity
+    G_rfft(R) = exp(2πi ΣR/Ns) G_fft(R) that an even G implies."""
+    kG, sP_, mP, Σ_loc, gLoc, gf, gr, rng = _tiny_inputs(D=D, Ns=Ns)
+    f, r = O.G_fft_from_wedge(gLoc, kG)
+    assert np.allclose(f, gf, rtol=0, atol=1e-13) and np.allclose(r, gr, rtol=0, atol=1e-13)
+    Gw = rng.standard_normal((kG.Nq, 3)) + 1j * rng.standard_normal((kG.Nq, 3))
+    f, r = O.G_fft_from_wedge(Gw, kG)
+    R = np.stack(np.meshgrid(*([np.arange(Ns)] * D), indexing="ij"), -1).reshape(-1, D)
+    E = np.exp(-2j * np.pi * (R @ R.T) / Ns)          # E[R, k] = exp(-2πi k.R/Ns)
+    for i in range(3):
+        g = kG.expand(Gw[:, i]).reshape(-1)
+        grev = np.array([kG.expand(Gw[:, i])[tuple(Ns - 1 - k)] for k in R])
+        assert np.allclose(f[..., i].reshape(-1), E @ g, atol=1e-12)
+        assert np.allclose(r[..., i].reshape(-1), E @ grev, atol=1e-12)
+    shift = np.exp(2j * np.pi * R.sum(axis=1) / Ns).reshape((Ns,) * D)
+    assert np.allclose(r, f * shift[..., None], atol=1e-12)
+
+
 def test_bubble_local_identity():
     """kintegrate_q chi0(q,nu,w) = -beta G_loc(nu) G_loc(nu+w) (normalisation of the convolution)."""
     kG, sP_, mP, Σ_loc, gLoc, gf, gr, _ = _tiny_inputs(Ns=6)
