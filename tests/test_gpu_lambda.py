@@ -73,6 +73,25 @@ def test_lambdadm_residual_parity(ctx, name, λ):
     assert np.max(np.abs(G - ref["G"])) <= 1e-10 * np.max(np.abs(ref["G"]))
 
 
+def test_self_consistency_rebuild(ctx):
+    """One rebuild of the self-consistency loop (conditions_new.jl:485-492) on the GPU: G_ladder from the
+    λdm residual, its G tables (helpers.jl:153-183) and the next bubble, against the oracle's chain."""
+    import ladderdga_hip as L
+    name, λ = "tiny", (94.0, -26.0)
+    print(f"self-consistency rebuild {name} {λ}", flush=True)
+    inp, oK, omP, osP, o, p, νmax = ladder(name)
+    prob = problem(ctx, name)
+    got = prob.eval(λ, arrays=True)
+    prob.free()
+    ref = oracle_res(name, λ)
+    rel = lambda a, b: float(np.max(np.abs(np.asarray(a) - b)) / np.max(np.abs(b)))
+    f, r = L.G_fft(np.asfortranarray(got["G_ladder"]), inp.kG, inp.sP, ctx=ctx)
+    rf, rr = O.G_fft_from_wedge(ref["G"], oK)
+    assert rel(f, rf) <= 1e-10 and rel(r, rr) <= 1e-10
+    χ0 = L.calc_bubble(f, r, inp.kG, inp.mP, inp.sP, ctx=ctx)
+    assert rel(χ0.data, O.calc_bubble(rf, rr, oK, omP, osP)) <= 1e-10
+
+
 @pytest.mark.parametrize("name,λ", [("tiny", (94.0, -26.0)), ("tiny", (130.0, -10.0)), ("tiny", (40.0, 5.0))])
 def test_lambdadm_adjoint_jacobian(ctx, name, λ):
     """Adjoint Jacobian vs central differences (h = 1e-6 relative) of the oracle residual."""
