@@ -215,6 +215,21 @@ int ldga_lambdadm_residual_dev(ldga_ctx *ctx, const ldga_freq *fr, const ldga_mo
                                const void *G_rfft, const void *Sigma_loc, int want_jac, ldga_lambdadm_out *out,
                                void *sigma, void *G_ladder, void *stream);
 
+/* ---- G_from_Σladder! and calc_E on a given Sigma_ladder ------------------
+ * (GFTools.jl:162-186, thermodynamics.jl:142-168): the step of the
+ * self-consistency loop after the EoM (conditions_new.jl:484-488), device
+ * pointers.  sigma (Nq x numax) is in/out: with sigma_prev != NULL and
+ * mixing != 0 it is first replaced by (1 - mixing) sigma + mixing sigma_prev
+ * (conditions_new.jl:484).  Then mu is refit (fit_mu != 0) so that
+ * filling_pos(G_ladder[:, 0:numax-1]) = n, G_ladder (Nq x n_fft, nullable) is
+ * written, and `out` carries n, mu, E_kin, E_pot1 and, from chi_m / chi_d at
+ * (lam_m, lam_d), lhs_c1, E_pot2 and the two residuals (jac is not filled).
+ * Sigma_loc is read only for G_ladder.  Synchronises the stream. */
+int ldga_g_from_sigma_ladder_dev(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m,
+                                 double lam_d, int fit_mu, double mu0, const void *chi_m, const void *chi_d,
+                                 void *sigma, const void *sigma_prev, double mixing, const void *Sigma_loc,
+                                 ldga_lambdadm_out *out, void *G_ladder, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

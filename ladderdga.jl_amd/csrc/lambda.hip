@@ -112,6 +112,43 @@ __global__ __launch_bounds__(kFT) void lambdadm_final_kernel(OctTables T, FinalA
       f1 = filling(x1, sumG(x1)) - n;
     }
     mu = x1;
+    if (!conv) {
+      // fallback, as a bracketing solver would (the filling grows with mu): step
+      // out from mu0 until the sign changes, then Illinois false position
+      double a = A.mu0, fa = filling(a, sumG(a)) - n;
+      const double dir = fa < 0.0 ? 1.0 : -1.0;
+      double h = 1.0, b = a + dir * h, fb = filling(b, sumG(b)) - n;
+      for (int k = 0; k < 60 && !(fa * fb <= 0.0); ++k) {
+        a = b;
+        fa = fb;
+        h *= 2.0;
+        b = a + dir * h;
+        fb = filling(b, sumG(b)) - n;
+      }
+      if (fa * fb <= 0.0) {
+        int side = 0;
+        for (int k = 0; k < 200; ++k, ++iters) {
+          const double c = (a * fb - b * fa) / (fb - fa);
+          const double fc = filling(c, sumG(c)) - n;
+          mu = c;
+          if (fabs(fc) < A.tol || fabs(b - a) <= 4e-16 * fmax(fabs(a), fabs(b))) {
+            conv = 1;
+            break;
+          }
+          if (fc * fb > 0.0) {
+            b = c;
+            fb = fc;
+            if (side == -1) fa *= 0.5;
+            side = -1;
+          } else {
+            a = c;
+            fa = fc;
+            if (side == 1) fb *= 0.5;
+            side = 1;
+          }
+        }
+      }
+    }
   }
   // ---- filling, calc_E (thermodynamics.jl:142-168), and for the Jacobian
   //      d(filling)/dmu and dE_pot/dmu
@@ -192,6 +229,12 @@ __global__ __launch_bounds__(kFT) void lambdadm_final_kernel(OctTables T, FinalA
 
 // G_ladder over fft_range: Sigma_ladder below numax, Sigma_loc above, negative
 // frequencies by conjugation (GFTools.jl:162-186)
+// sigma <- (1 - m) sigma + m sigma_prev (conditions_new.jl:484)
+__global__ void mix_kernel(double2 *__restrict__ sigma, const double2 *__restrict__ prev, double m, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) sigma[i] = cadd(cscale(sigma[i], 1.0 - m), cscale(prev[i], m));
+}
+
 __global__ void gladder_kernel(OctTables T, int numax, int fft_first, int n_fft, double beta, const double *res,
                                const double2 *__restrict__ sigma, const double2 *__restrict__ sigloc,
                                double2 *__restrict__ G) {
@@ -232,8 +275,12 @@ static int lambdadm_dev(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp
                         double lam_d, int tc, double chiloc_m_sum, int fit_mu, double mu0, const double *chi_m,
                         const double2 *gamma_m, const double *chi_d, const double2 *gamma_d, const double2 *lambda0,
                         const double2 *Grfft, const double2 *sigloc, int want_jac, ldga_lambdadm_out *out,
-                        double2 *sigma_out, double2 *G_out, hipStream_t st) {
+                        double2 *sigma_out, double2 *G_out, hipStream_t st, bool eom = true,
+                        const double2 *sigma_prev = nullptr, double mixing = 0.0) {
+  // eom = false: sigma_out holds a given Sigma_ladder (optionally mixed with
+  // sigma_prev first) and only the mu fit, the energies and G_ladder follow
   if (!out) return ldga_fail(ctx, LDGA_ERR_ARG, "out is NULL");
+  if (!eom && (!sigma_out || want_jac)) return ldga_fail(ctx, LDGA_ERR_ARG, "a given Sigma needs sigma and no Jacobian");
   if (!ctx->has_kgrid) return ldga_fail(ctx, LDGA_ERR_STATE, "ldga_set_kgrid has not been called");
   if (G_out && !sigloc) return ldga_fail(ctx, LDGA_ERR_ARG, "G_ladder needs Sigma_loc");
   const int Nq = ctx->Nq, Nw = 2 * fr->n_iw + 1;
@@ -247,9 +294,16 @@ static int lambdadm_dev(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp
   double2 *sigma = sigma_out ? sigma_out : (double2 *)ldga_buf(ctx, "ldm_sigma", (size_t)Nq * numax * 16);
   double *res = (double *)ldga_buf(ctx, "ldm_res", 16 * 8 + 2 * 16);
   if (!sigma || !res) return LDGA_ERR_HIP;
-  int rc = launch_sigma(ctx, fr, mp, numax, lam_m, lam_d, tc, LDGA_EOM_FULL, chiloc_m_sum, chi_m, gamma_m, chi_d,
-                        gamma_d, lambda0, Grfft, sigma, st);
-  if (rc) return rc;
+  int rc = LDGA_OK;
+  if (eom) {
+    rc = launch_sigma(ctx, fr, mp, numax, lam_m, lam_d, tc, LDGA_EOM_FULL, chiloc_m_sum, chi_m, gamma_m, chi_d,
+                      gamma_d, lambda0, Grfft, sigma, st);
+    if (rc) return rc;
+  } else if (sigma_prev && mixing != 0.0) {
+    const size_t tot = (size_t)Nq * numax;
+    mix_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(sigma, sigma_prev, mixing, tot);
+    LDGA_HIP_TRY(ctx, hipGetLastError());
+  }
   FinalArgs A{};
   A.sigma = sigma;
   A.chim = chi_m;
@@ -384,6 +438,20 @@ int ldga_lambdadm_residual(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model 
   if (sigma) LDGA_HIP_TRY(ctx, hipMemcpy(sigma, so, Nq * (size_t)numax * 16, hipMemcpyDeviceToHost));
   if (G_ladder) LDGA_HIP_TRY(ctx, hipMemcpy(G_ladder, go, Nq * (size_t)fr->n_fft * 16, hipMemcpyDeviceToHost));
   return LDGA_OK;
+}
+
+int ldga_g_from_sigma_ladder_dev(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m,
+                                 double lam_d, int fit_mu, double mu0, const void *chi_m, const void *chi_d,
+                                 void *sigma, const void *sigma_prev, double mixing, const void *Sigma_loc,
+                                 ldga_lambdadm_out *out, void *G_ladder, void *stream) {
+  if (!ctx || !mp || !chi_m || !chi_d || !sigma) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  int rc = ldga_check_freq(ctx, fr);
+  if (rc) return rc;
+  hipSetDevice(ctx->device);
+  return lambdadm_dev(ctx, fr, mp, numax, lam_m, lam_d, 0, 0.0, fit_mu, mu0, (const double *)chi_m, nullptr,
+                      (const double *)chi_d, nullptr, nullptr, nullptr, (const double2 *)Sigma_loc, 0, out,
+                      (double2 *)sigma, (double2 *)G_ladder, ldga_stream(ctx, stream), false,
+                      (const double2 *)sigma_prev, mixing);
 }
 
 }  // extern "C"

@@ -35,7 +35,7 @@ EXPORTED = [
     "ldga_sum_kw", "ldga_sum_kw_dev",
     "ldga_sigma", "ldga_sigma_dev",
     "ldga_malloc", "ldga_free", "ldga_memcpy_h2d", "ldga_memcpy_d2h",
-    "ldga_G_shell_sum", "ldga_lambdadm_residual", "ldga_lambdadm_residual_dev",
+    "ldga_G_shell_sum", "ldga_lambdadm_residual", "ldga_lambdadm_residual_dev", "ldga_g_from_sigma_ladder_dev",
 ]
 
 
@@ -112,6 +112,8 @@ def lib():
             "ldga_lambdadm_residual_dev": (i32, [vp, C.POINTER(Freq), C.POINTER(Model), i32, dbl, dbl, i32, dbl,
                                                  i32, dbl, vp, vp, vp, vp, vp, vp, vp, i32, C.POINTER(LambdaDmOut),
                                                  vp, vp, vp]),
+            "ldga_g_from_sigma_ladder_dev": (i32, [vp, C.POINTER(Freq), C.POINTER(Model), i32, dbl, dbl, i32, dbl,
+                                                   vp, vp, vp, vp, dbl, vp, C.POINTER(LambdaDmOut), vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -15,7 +15,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import LambdaDmOut, default_context
-from .api import _freq, _model, get_Î»_min
+from .api import _freq, _model, get_Î»_min, sum_kÏ‰
 from .kgrid import KGrid
 from .types import ModelParameters, SimulationParameters, Ï‡T, Î³T
 
@@ -253,9 +253,124 @@ def Î»dm_correction(Ï‡m: Ï‡T, Î³m: Î³T, Ï‡d: Ï‡T, Î³d: Î³T, Î£_loc, gLoc_rfft, Ï
         if Î»_val_only:
             return root[0], root[1], converged
         Gl = r["G_ladder"]
-        return Î»_result(root[0], root[1], "dm", True, converged, r["Ekin"], r["Epot1"], r["Epot2"], rhs_c1,
-                        r["lhs_c1"], [], Gl, r["Î£_ladder"], r["mu"], r["n"], steps, prob.evals)
+        # the reference's constructor stores its 4th/5th arguments swapped (conditions_new.jl:42):
+        # `converged` is the sum-rule validation, `sc_converged` is true
+        return Î»_result(root[0], root[1], "dm", converged=converged, sc_converged=True, EKin=r["Ekin"],
+                        EPot_p1=r["Epot1"], EPot_p2=r["Epot2"], PP_p1=rhs_c1, PP_p2=r["lhs_c1"], trace=[],
+                        G_ladder=Gl, Î£_ladder=r["Î£_ladder"], Î¼=r["mu"], n=r["n"], newton_steps=steps,
+                        residual_evals=prob.evals)
     prob.free()
     if Î»_val_only:
         return root[0], root[1], False
-    return Î»_result(root[0], root[1], "dm", True, False, newton_steps=steps, residual_evals=prob.evals)
+    return Î»_result(root[0], root[1], "dm", converged=False, sc_converged=True, newton_steps=steps,
+                    residual_evals=prob.evals)
+
+
+# ---------------------------------------------------------------------------
+# self-consistency loop (LambdaCorrection/conditions_new.jl:342-401, run_sc! :405-560)
+# ---------------------------------------------------------------------------
+def run_sc(Ï‡m: Ï‡T, Î³m: Î³T, Ï‡d: Ï‡T, Î³d: Î³T, Î»0, Î¼: float, Î£_loc, gLoc_rfft, Ï‡loc_m_sum, kG: KGrid,
+           mP: ModelParameters, sP: SimulationParameters, type: str = "fix", Î½max: int = -1, maxit: int = 100,
+           mixing: float = 0.2, conv_abs: float = 1e-8, fit_Î¼: bool = True, tc: bool = True, Î»m: float = 0.0,
+           Î»d: float = 0.0, Î»_min_Î´: float = 1e-4, update_Ï‡_tail: bool = False, trace: bool = False,
+           ctx=None) -> Î»_result:
+    """Self-consistent lDÎ“A with the loop on the GPU.
+
+    Every iteration runs, with the arrays resident on the device: the EoM with
+    the current G_rfft (calc_Î£!, :479), mixing with the previous Î£ (:484),
+    G_from_Î£ladder! with the Î¼ fit and calc_E (:485-490), and the rebuild of
+    G_rfft from G_ladder (G_rfft!, :491).  type "fix" keeps (Î»m, Î»d); "dm"
+    starts from them and re-solves Î»dm_correction with the current G_rfft in
+    every later iteration (:437-449); "pre_dm" first solves Î»dm_correction
+    (:370-372).  The loop stops when both potential energies change by less
+    than conv_abs, or after maxit iterations.  Ï‡m and Ï‡d are left unchanged.
+    """
+    import warnings
+    if type not in ("fix", "dm", "pre_dm"):
+        raise ValueError(f"run_sc type {type!r}: supported are 'fix', 'dm' and 'pre_dm'")
+    if update_Ï‡_tail:
+        raise NotImplementedError("update_Ï‡_tail (the Ï‡ tail self-consistency) is not supported")
+    if not fit_Î¼:
+        warnings.warn("Not fitting Î¼ can lead to unphysical results!")
+    if Ï‡m.Î» != 0 or Ï‡d.Î» != 0:
+        raise RuntimeError("Î» parameter already set; run_sc applies Î»m, Î»d itself")
+    _, Î½max_d = gen_Î½Ï‰_indices(Ï‡m, Ï‡d, sP)
+    Î½max = Î½max_d if Î½max < 1 else min(Î½max_d, Î½max)
+    validation = True
+    if type == "pre_dm":
+        Î»m, Î»d, validation = Î»dm_correction(Ï‡m, Î³m, Ï‡d, Î³d, Î£_loc, gLoc_rfft, Ï‡loc_m_sum, Î»0, kG, mP, sP,
+                                            Î½max=Î½max, Î»_min_Î´=Î»_min_Î´, Î»_val_only=True, fit_Î¼=fit_Î¼,
+                                            validate_threshold=conv_abs, tc=tc, ctx=ctx)
+    if not validation or not (math.isfinite(Î»m) and math.isfinite(Î»d)):
+        return Î»_result(Î»m, Î»d, type, converged=False, sc_converged=False)
+
+    ctx = ctx or default_context()
+    ctx.set_kgrid(kG)
+    L = _lib.lib()
+    up = ctx.to_device
+    nf = len(sP.fft_range)
+    d = dict(Ï‡m=up(np.asarray(Ï‡m.data, np.float64)), Ï‡d=up(np.asarray(Ï‡d.data, np.float64)),
+             Î³m=up(np.asarray(Î³m.data, np.complex128)), Î³d=up(np.asarray(Î³d.data, np.complex128)),
+             Î»0=up(np.asarray(Î»0, np.complex128)), Gr=up(np.asarray(gLoc_rfft, np.complex128)),
+             Î£loc=up(np.ascontiguousarray(Î£_loc[:sP.fft_range.stop], np.complex128)))
+    d["Gf"] = ctx.empty(d["Gr"].shape, np.complex128)                  # G_fft of the rebuild (unused)
+    d["G"] = ctx.empty((kG.Nq, nf), np.complex128)
+    cur, nxt = ctx.empty((kG.Nq, Î½max), np.complex128), ctx.empty((kG.Nq, Î½max), np.complex128)
+    fr, mo = _freq(sP), _model(mP)
+    out = LambdaDmOut()
+    # the sums at (Î»m, Î»d) before the loop (run_sc! :417-421)
+    Ï‡m_sum = sum_kÏ‰(kG, Ï‡m, Î»=Î»m, ctx=ctx)
+    Ï‡d_sum = sum_kÏ‰(kG, Ï‡d, Î»=Î»d, ctx=ctx)
+    lhs_c1 = (Ï‡d_sum + Ï‡m_sum) / 2
+    rhs_c1 = mP.n / 2 * (1 - mP.n / 2)
+    E_pot_2 = (mP.U / 2) * (Ï‡d_sum - Ï‡m_sum) + mP.U * (mP.n / 2 * mP.n / 2)
+    E_pot_1 = E_kin = math.inf
+    rows = []
+    it, converged, n = 1, False, math.nan
+    try:
+        while True:
+            if type in ("dm", "pre_dm") and it > 1:
+                r = Î»dm_correction(Ï‡m, Î³m, Ï‡d, Î³d, Î£_loc, d["Gr"].to_host(), Ï‡loc_m_sum, Î»0, kG, mP, sP,
+                                   Î½max=Î½max, Î»_min_Î´=Î»_min_Î´, fit_Î¼=fit_Î¼, Î»init=(Î»m, Î»d), tc=tc, ctx=ctx)
+                Î»m, Î»d, validation = r.Î»m, r.Î»d, r.converged
+                if math.isfinite(r.Î¼):
+                    Î¼ = r.Î¼
+            if not (math.isfinite(Î»m) and math.isfinite(Î»d)) or not validation:
+                print(f"ERROR: Î»m = {Î»m} or Î»d = {Î»d} not finite, OR internal Î» correction did not find root!")
+                break
+            ctx.check(L.ldga_sigma_dev(ctx.handle, C.byref(fr), C.byref(mo), Î½max, Î»m, Î»d, int(tc), 0,
+                                       float(Ï‡loc_m_sum), d["Ï‡m"].ptr, d["Î³m"].ptr, d["Ï‡d"].ptr, d["Î³d"].ptr,
+                                       d["Î»0"].ptr, d["Gr"].ptr, nxt.ptr, None))
+            prev = cur.ptr if (mixing != 0 and it > 1) else None
+            ctx.check(L.ldga_g_from_sigma_ladder_dev(ctx.handle, C.byref(fr), C.byref(mo), Î½max, Î»m, Î»d,
+                                                     int(fit_Î¼), Î¼, d["Ï‡m"].ptr, d["Ï‡d"].ptr, nxt.ptr, prev,
+                                                     float(mixing), d["Î£loc"].ptr, C.byref(out), d["G"].ptr, None))
+            cur, nxt = nxt, cur
+            if math.isnan(out.mu):
+                break
+            Î¼, n = out.mu, out.n
+            E_pot_1_old, E_pot_2_old = E_pot_1, E_pot_2
+            E_kin, E_pot_1 = out.Ekin, out.Epot1
+            ctx.check(L.ldga_g_tables_dev(ctx.handle, C.byref(fr), d["G"].ptr, d["Gf"].ptr, d["Gr"].ptr, None))
+            if type in ("dm", "pre_dm"):
+                lhs_c1, E_pot_2 = out.lhs_c1, out.Epot2
+            if trace:
+                rows.append(dict(it=it, Î»m=Î»m, Î»d=Î»d, Î¼=Î¼, n=n, EKin=E_kin, EPot=E_pot_1, lhs_c1=lhs_c1,
+                                 EPot_c2=E_pot_2))
+            if abs(E_pot_1 - E_pot_1_old) < conv_abs and abs(E_pot_2 - E_pot_2_old) < conv_abs:
+                converged = True
+                break
+            if it >= maxit:
+                break
+            it += 1
+        Î£_ladder, G_ladder = cur.to_host(), d["G"].to_host()
+    finally:
+        for a in list(d.values()) + [cur, nxt]:
+            a.free()
+    ok = (converged and math.isfinite(lhs_c1) and math.isfinite(E_pot_2) and abs(rhs_c1 - lhs_c1) <= conv_abs
+          and abs(E_pot_1 - E_pot_2) <= conv_abs)
+    # the reference's constructor stores its 4th/5th arguments swapped (conditions_new.jl:42):
+    # `converged` is the sum-rule validation, `sc_converged` the loop's convergence
+    return Î»_result(Î»m, Î»d, type, converged=ok, sc_converged=converged, EKin=E_kin, EPot_p1=E_pot_1,
+                    EPot_p2=E_pot_2, PP_p1=rhs_c1, PP_p2=lhs_c1, trace=rows, G_ladder=G_ladder, Î£_ladder=Î£_ladder,
+                    Î¼=Î¼, n=n)

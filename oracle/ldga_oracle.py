@@ -608,16 +608,53 @@ def G_from_Σladder(Σ_ladder, Σ_loc, kG, mP, sP, fix_n=True, μ=None, secant_t
     return μ, G
 
 
-def secant(f, x0, tol=1e-13, maxit=200):
+def secant(f, x0, tol=1e-13, maxit=100):
+    """Root of the filling condition: a secant from x0; if it does not reach
+    |f| < tol, the fallback of a bracketing solver (the filling grows with μ):
+    step out from x0 until the sign changes, then Illinois false position."""
+    xs = x0
     x1 = x0 + 1e-4 if x0 == 0 else x0 * (1 + 1e-4)
     f0, f1 = f(x0), f(x1)
     for _ in range(maxit):
-        if abs(f1) < tol or f1 == f0:
+        if abs(f1) < tol:
             return x1
+        if f1 == f0:
+            break
         x2 = x1 - f1 * (x1 - x0) / (f1 - f0)
         x0, f0, x1 = x1, f1, x2
         f1 = f(x1)
-    return x1
+    a = xs
+    fa = f(a)
+    d = 1.0 if fa < 0 else -1.0
+    h = 1.0
+    b = a + d * h
+    fb = f(b)
+    k = 0
+    while not (fa * fb <= 0) and k < 60:
+        a, fa = b, fb
+        h *= 2.0
+        b = a + d * h
+        fb = f(b)
+        k += 1
+    if not (fa * fb <= 0):
+        return x1
+    side, c = 0, x1
+    for _ in range(200):
+        c = (a * fb - b * fa) / (fb - fa)
+        fc = f(c)
+        if abs(fc) < tol or abs(b - a) <= 4e-16 * max(abs(a), abs(b)):
+            return c
+        if fc * fb > 0:
+            b, fb = c, fc
+            if side == -1:
+                fa *= 0.5
+            side = -1
+        else:
+            a, fa = c, fc
+            if side == 1:
+                fb *= 0.5
+            side = 1
+    return c
 
 
 def calc_E(G, Σ, μ, kG, mP, sP):
@@ -667,6 +704,46 @@ def λdm_residual_vals(λ, χm, γm, χd, γd, Σ_loc, g_rfft, χloc_m_sum, λ0,
     lhs_c1 = (χd_sum + χm_sum) / 2
     E_pot_2 = (mP.U / 2) * (χd_sum - χm_sum) + mP.U * (mP.n / 2 * mP.n / 2)
     return n, μ, E_kin, E_pot_1, E_pot_2, lhs_c1, Σ, G
+
+
+def run_sc_fix(χm, γm, χd, γd, λ0, μ, Σ_loc, g_rfft, χloc_m_sum, kG, mP, sP, νmax, λm, λd,
+               maxit=100, mixing=0.2, conv_abs=1e-8, fit_μ=True, tc=True):
+    """run_sc! (LambdaCorrection/conditions_new.jl:405-560) for type :fix: λ fixed, no χ-tail update.
+
+    Each iteration: EoM with the current G_rfft (:479), mixing with the previous Σ (:484),
+    G_from_Σladder! with the μ fit (:485), calc_E (:490) and the G_rfft! rebuild (:491);
+    converged when both potential energies change by less than conv_abs (:503).
+    """
+    f0 = sP.fft_range.start
+    χm_l = ChiT(χ_λ(χm.data, λm), χm.beta, tail_c=χm.tail_c, usable_ω=χm.usable_ω)
+    χd_l = ChiT(χ_λ(χd.data, λd), χd.beta, tail_c=χd.tail_c, usable_ω=χd.usable_ω)
+    χm_sum, χd_sum = sum_kω(kG, χm_l), sum_kω(kG, χd_l)
+    lhs_c1 = (χd_sum + χm_sum) / 2
+    rhs_c1 = mP.n / 2 * (1 - mP.n / 2)
+    E_pot_2 = (mP.U / 2) * (χd_sum - χm_sum) + mP.U * (mP.n / 2 * mP.n / 2)
+    E_pot_1, E_kin = math.inf, math.inf
+    g_rfft = np.array(g_rfft, copy=True)
+    Σ, G, it, converged = None, None, 1, False
+    while True:
+        Σ_work = Σ
+        Σ = calc_Σ(χm, γm, χd, γd, χloc_m_sum, λ0, g_rfft, kG, mP, sP, νmax=νmax, λm=λm, λd=λd, tc=tc)
+        if mixing != 0 and it > 1:
+            Σ = (1 - mixing) * Σ + mixing * Σ_work
+        μ, G = G_from_Σladder(Σ, Σ_loc, kG, mP, sP, fix_n=fit_μ, μ=μ)
+        if math.isnan(μ):
+            break
+        E_pot_1_old, E_pot_2_old = E_pot_1, E_pot_2
+        E_kin, E_pot_1 = calc_E(G, Σ, μ, kG, mP, sP)
+        _, g_rfft = G_fft_from_wedge(G, kG)
+        if abs(E_pot_1 - E_pot_1_old) < conv_abs and abs(E_pot_2 - E_pot_2_old) < conv_abs:
+            converged = True
+            break
+        if it >= maxit:
+            break
+        it += 1
+    n = filling_pos_kG(G[:, -f0:-f0 + νmax], kG, mP.U, μ, mP.beta)
+    return dict(μ=μ, n=n, E_kin=E_kin, E_pot_1=E_pot_1, E_pot_2=E_pot_2, lhs_c1=lhs_c1, rhs_c1=rhs_c1,
+                sc_converged=converged, it=it, Σ=Σ, G=G)
 
 
 def newton_right_1d(f, df, start, min_, nsteps=100, atol=1e-13):

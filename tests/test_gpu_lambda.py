@@ -92,6 +92,63 @@ def test_self_consistency_rebuild(ctx):
     assert rel(χ0.data, O.calc_bubble(rf, rr, oK, omP, osP)) <= 1e-10
 
 
+def λdm_root(name):
+    """The oracle's λdm root (cached)."""
+    key = ("root", name)
+    if key not in _cache:
+        inp, oK, omP, osP, o, p, νmax = ladder(name)
+        _cache[key] = tuple(O.λdm_correction(o["χm"], o["γm"], o["χd"], o["γd"], inp.Σ_loc, inp.G_rfft,
+                                             inp.χloc_m_sum, o["λ0"], oK, omP, osP)[0])
+    return _cache[key]
+
+
+@pytest.mark.parametrize("at,mixing,maxit", [("fixed", 0.2, 6), ("fixed", 0.0, 4), ("root", 0.5, 6)])
+def test_run_sc_fix(ctx, at, mixing, maxit):
+    """run_sc (conditions_new.jl:342-560, type :fix) with the loop on the GPU against the oracle's loop:
+    the same iterations, μ, filling, energies, Σ_ladder and G_ladder, at an arbitrary λ and at the λdm
+    root.  On these synthetic inputs the loop does not converge for any mixing (the random vertices are
+    not a self-consistent solution), so the iteration amplifies rounding and only the first few steps
+    are comparable; the loop length here keeps the two trajectories within 1e-9."""
+    import ladderdga_hip as L
+    name = "tiny"
+    λ = (94.0, -26.0) if at == "fixed" else λdm_root(name)
+    print(f"run_sc fix λ={λ} mixing={mixing} maxit={maxit}", flush=True)
+    inp, oK, omP, osP, o, p, νmax = ladder(name)
+    r = L.run_sc(p["χm"], p["γm"], p["χd"], p["γd"], o["λ0"], inp.mP.μ, inp.Σ_loc, inp.G_rfft, inp.χloc_m_sum,
+                 inp.kG, inp.mP, inp.sP, type="fix", νmax=νmax, maxit=maxit, mixing=mixing, λm=λ[0], λd=λ[1],
+                 trace=True, ctx=ctx)
+    ref = O.run_sc_fix(o["χm"], o["γm"], o["χd"], o["γd"], o["λ0"], omP.mu, inp.Σ_loc, inp.G_rfft, inp.χloc_m_sum,
+                       oK, omP, osP, νmax, λ[0], λ[1], maxit=maxit, mixing=mixing)
+    assert len(r.trace) == ref["it"] and r.sc_converged == ref["sc_converged"]
+    for a, b in ((r.μ, ref["μ"]), (r.n, ref["n"]), (r.EKin, ref["E_kin"]), (r.EPot_p1, ref["E_pot_1"]),
+                 (r.EPot_p2, ref["E_pot_2"]), (r.PP_p2, ref["lhs_c1"])):
+        assert a == pytest.approx(b, rel=1e-9, abs=1e-11)
+    rel = lambda a, b: float(np.max(np.abs(np.asarray(a) - b)) / np.max(np.abs(b)))
+    assert rel(r.Σ_ladder, ref["Σ"]) <= 1e-9 and rel(r.G_ladder, ref["G"]) <= 1e-9
+    # χ is left untouched
+    assert np.array_equal(p["χm"].data, np.asfortranarray(o["χm"].data))
+
+
+def test_run_sc_pre_dm(ctx):
+    """run_sc type :pre_dm: λ from λdm_correction first (the oracle's root), then re-solved with every
+    rebuilt G_rfft.  The first iteration runs at the root; on these synthetic inputs a later re-solve
+    finds no root, and the loop then stops with λ = NaN and converged = false, as the reference's does
+    (conditions_new.jl:465-471)."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP, o, p, νmax = ladder("tiny")
+    print("run_sc pre_dm", flush=True)
+    r = L.run_sc(p["χm"], p["γm"], p["χd"], p["γd"], o["λ0"], inp.mP.μ, inp.Σ_loc, inp.G_rfft, inp.χloc_m_sum,
+                 inp.kG, inp.mP, inp.sP, type="pre_dm", νmax=νmax, maxit=6, mixing=0.2, trace=True, ctx=ctx)
+    assert 1 <= len(r.trace) <= 6
+    first = r.trace[0]
+    assert np.allclose([first["λm"], first["λd"]], λdm_root("tiny"), rtol=1e-7, atol=1e-7)
+    assert np.isfinite([first["μ"], first["n"], first["EPot"], first["EPot_c2"]]).all()
+    if not (np.isfinite(r.λm) and np.isfinite(r.λd)):
+        assert not r.converged and not r.sc_converged
+    elif r.sc_converged:
+        assert r.converged == (abs(r.PP_p1 - r.PP_p2) <= 1e-8 and abs(r.EPot_p1 - r.EPot_p2) <= 1e-8)
+
+
 @pytest.mark.parametrize("name,λ", [("tiny", (94.0, -26.0)), ("tiny", (130.0, -10.0)), ("tiny", (40.0, 5.0))])
 def test_lambdadm_adjoint_jacobian(ctx, name, λ):
     """Adjoint Jacobian vs central differences (h = 1e-6 relative) of the oracle residual."""

@@ -213,6 +213,15 @@ def test_G_fft_from_wedge(D, Ns):
     assert np.allclose(r, f * shift[..., None], atol=1e-12)
 
 
+def test_mu_root_fallback():
+    """The μ root: where a plain secant from x0 runs off (atan(x - 5) from 0 diverges to ~1e20), the
+    bracketing fallback (step out, then Illinois) still finds the root; a well-behaved case is
+    solved by the secant itself."""
+    assert O.secant(lambda x: math.atan(x - 5.0), 0.0) == pytest.approx(5.0, abs=1e-12)
+    assert O.secant(lambda x: math.atan(x + 3.0), 2.0) == pytest.approx(-3.0, abs=1e-12)
+    assert O.secant(lambda x: x ** 3 + x - 2.0, 0.9) == pytest.approx(1.0, abs=1e-12)
+
+
 def test_bubble_local_identity():
     """kintegrate_q chi0(q,nu,w) = -beta G_loc(nu) G_loc(nu+w) (normalisation of the convolution)."""
     kG, sP_, mP, Σ_loc, gLoc, gf, gr, _ = _tiny_inputs(Ns=6)
