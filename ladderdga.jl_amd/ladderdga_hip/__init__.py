@@ -8,10 +8,10 @@ from .types import ModelParameters, SimulationParameters, χ0T, χT, γT  # noqa
 from .api import (G_fft, calc_bubble, calc_χγ, calc_λ0, calc_Σ, calc_Σ_parts, χ_λ, dχ_λ, χ_λ_inplace, reset,  # noqa: F401
                   sum_kω, get_λ_min)
 
-from .lambda_correction import (λ_result, newton_right, newton_right_1d, λm_correction, λdm_correction, run_sc,  # noqa: F401
+from .lambda_correction import (λ_result, newton_right, newton_right_1d, λm_rhs, λm_correction, λdm_correction, run_sc,  # noqa: F401
                                 LambdaDmProblem, gen_νω_indices)
 
-__all__ = ["λ_result", "newton_right", "newton_right_1d", "λm_correction", "λdm_correction", "run_sc", "LambdaDmProblem",
+__all__ = ["λ_result", "newton_right", "newton_right_1d", "λm_rhs", "λm_correction", "λdm_correction", "run_sc", "LambdaDmProblem",
            "gen_νω_indices", "Context", "LdgaError", "KGrid", "gen_kGrid", "ModelParameters", "SimulationParameters", "χ0T", "χT",
            "γT", "G_fft", "calc_bubble", "calc_χγ", "calc_λ0", "calc_Σ", "calc_Σ_parts", "χ_λ", "dχ_λ", "χ_λ_inplace",
            "reset", "sum_kω", "get_λ_min"]

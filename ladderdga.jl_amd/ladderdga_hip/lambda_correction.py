@@ -152,6 +152,26 @@ def λm_correction(χm: χT, rhs: float, kG: KGrid, mP: ModelParameters, sP: Sim
     return λm, bool(abs(rhs - check) <= validate_threshold)
 
 
+def λm_rhs(imp_density: float, χm: χT, χd: χT, kG: KGrid, mP: ModelParameters, sP: SimulationParameters,
+           λd: float = math.nan, λ_rhs: str = "native", ctx=None) -> float:
+    """Right-hand side of the λm condition (LambdaCorrection/helpers.jl:234-255).
+
+    "fixed": n (1 - n/2) - Σ χd; "native" without a χ helper (plain mode):
+    2 imp_density - Σ χd, where imp_density is the impurity's equal-time
+    same-spin density correlator.  Σ χd is taken at λd when it is given.
+    """
+    if χd.λ != 0 and not math.isnan(λd):
+        raise RuntimeError(f"λd = {λd} given while χd.λ = {χd.λ} is set")
+    χd_sum = sum_kω(kG, χd, λ=λd, ctx=ctx)
+    if λ_rhs == "fixed":
+        return mP.n * (1 - mP.n / 2) - χd_sum
+    if λ_rhs != "native":
+        raise ValueError(f"λ_rhs {λ_rhs!r}: supported are 'native' and 'fixed'")
+    if not math.isfinite(imp_density):
+        raise ValueError("imp_density argument is not finite! Cannot use DMFT error compensation method")
+    return 2 * imp_density - χd_sum
+
+
 # ---------------------------------------------------------------------------
 # λdm (conditions_new.jl:219-324)
 # ---------------------------------------------------------------------------
@@ -273,7 +293,7 @@ def run_sc(χm: χT, γm: γT, χd: χT, γd: γT, λ0, μ: float, Σ_loc, gLoc_
            mP: ModelParameters, sP: SimulationParameters, type: str = "fix", νmax: int = -1, maxit: int = 100,
            mixing: float = 0.2, conv_abs: float = 1e-8, fit_μ: bool = True, tc: bool = True, λm: float = 0.0,
            λd: float = 0.0, λ_min_δ: float = 1e-4, update_χ_tail: bool = False, trace: bool = False,
-           ctx=None) -> λ_result:
+           imp_density: float = math.nan, ctx=None) -> λ_result:
     """Self-consistent lDΓA with the loop on the GPU.
 
     Every iteration runs, with the arrays resident on the device: the EoM with
@@ -282,12 +302,15 @@ def run_sc(χm: χT, γm: γT, χd: χT, γd: γT, λ0, μ: float, Σ_loc, gLoc_
     G_rfft from G_ladder (G_rfft!, :491).  type "fix" keeps (λm, λd); "dm"
     starts from them and re-solves λdm_correction with the current G_rfft in
     every later iteration (:437-449); "pre_dm" first solves λdm_correction
-    (:370-372).  The loop stops when both potential energies change by less
+    (:370-372).  "pre_m" first solves λm_correction with λm_rhs (plain mode:
+    2 imp_density - Σ χd) and sets λd = 0 (:365-369); the reference's later
+    iterations then keep λ fixed for "m" and "pre_m" (its `type == [:pre_m,
+    :m]` test at :450 never holds), so "m" runs as "fix".  The loop stops when both potential energies change by less
     than conv_abs, or after maxit iterations.  χm and χd are left unchanged.
     """
     import warnings
-    if type not in ("fix", "dm", "pre_dm"):
-        raise ValueError(f"run_sc type {type!r}: supported are 'fix', 'dm' and 'pre_dm'")
+    if type not in ("fix", "m", "pre_m", "dm", "pre_dm"):
+        raise ValueError(f"run_sc type {type!r}: supported are 'fix', 'm', 'pre_m', 'dm' and 'pre_dm'")
     if update_χ_tail:
         raise NotImplementedError("update_χ_tail (the χ tail self-consistency) is not supported")
     if not fit_μ:
@@ -297,6 +320,10 @@ def run_sc(χm: χT, γm: γT, χd: χT, γd: γT, λ0, μ: float, Σ_loc, gLoc_
     _, νmax_d = gen_νω_indices(χm, χd, sP)
     νmax = νmax_d if νmax < 1 else min(νmax_d, νmax)
     validation = True
+    if type == "pre_m":
+        rhs = λm_rhs(imp_density, χm, χd, kG, mP, sP, ctx=ctx)
+        λm, validation = λm_correction(χm, rhs, kG, mP, sP, ctx=ctx)
+        λd = 0.0
     if type == "pre_dm":
         λm, λd, validation = λdm_correction(χm, γm, χd, γd, Σ_loc, gLoc_rfft, χloc_m_sum, λ0, kG, mP, sP,
                                             νmax=νmax, λ_min_δ=λ_min_δ, λ_val_only=True, fit_μ=fit_μ,

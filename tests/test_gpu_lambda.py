@@ -149,6 +149,33 @@ def test_run_sc_pre_dm(ctx):
         assert r.converged == (abs(r.PP_p1 - r.PP_p2) <= 1e-8 and abs(r.EPot_p1 - r.EPot_p2) <= 1e-8)
 
 
+def test_run_sc_pre_m(ctx):
+    """run_sc type :pre_m (conditions_new.jl:365-369): λm from λm_correction with the plain-mode λm_rhs
+    (2 imp_density - Σ χd, helpers.jl:234-255), λd = 0, then the loop with λ kept fixed.  imp_density is
+    chosen so that the rhs is Σ χm at λ = 35.  The synthetic Σ χm^λ is not monotonic in λ and has several
+    roots; from this rhs the Newton path (12 steps, to the root near 76) is stable under 1e-13 changes of
+    the sums, while from other targets it wanders for 100 steps and ends where rounding sends it.  The
+    root must meet the condition, and the loop must follow the oracle's :fix loop at that λ."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP, o, p, νmax = ladder("tiny")
+    imp = (O.sum_kω(oK, o["χm"], λ=35.0) + O.sum_kω(oK, o["χd"])) / 2
+    rhs = 2 * imp - O.sum_kω(oK, o["χd"])
+    assert L.λm_rhs(imp, p["χm"], p["χd"], inp.kG, inp.mP, inp.sP, ctx=ctx) == pytest.approx(rhs, rel=1e-12)
+    λm = O.λm_correction(o["χm"], rhs, oK, omP, osP)
+    assert O.sum_kω(oK, o["χm"], λ=λm) == pytest.approx(rhs, rel=1e-10)
+    print(f"run_sc pre_m λm={λm}", flush=True)
+    r = L.run_sc(p["χm"], p["γm"], p["χd"], p["γd"], o["λ0"], inp.mP.μ, inp.Σ_loc, inp.G_rfft, inp.χloc_m_sum,
+                 inp.kG, inp.mP, inp.sP, type="pre_m", νmax=νmax, maxit=4, mixing=0.2, trace=True,
+                 imp_density=imp, ctx=ctx)
+    assert r.λm == pytest.approx(λm, rel=1e-9) and r.λd == 0.0
+    ref = O.run_sc_fix(o["χm"], o["γm"], o["χd"], o["γd"], o["λ0"], omP.mu, inp.Σ_loc, inp.G_rfft, inp.χloc_m_sum,
+                       oK, omP, osP, νmax, λm, 0.0, maxit=4, mixing=0.2)
+    assert len(r.trace) == ref["it"]
+    for a, b in ((r.μ, ref["μ"]), (r.EKin, ref["E_kin"]), (r.EPot_p1, ref["E_pot_1"]), (r.EPot_p2, ref["E_pot_2"])):
+        assert a == pytest.approx(b, rel=1e-8, abs=1e-10)
+    assert np.max(np.abs(r.Σ_ladder - ref["Σ"])) <= 1e-8 * np.max(np.abs(ref["Σ"]))
+
+
 @pytest.mark.parametrize("name,λ", [("tiny", (94.0, -26.0)), ("tiny", (130.0, -10.0)), ("tiny", (40.0, 5.0))])
 def test_lambdadm_adjoint_jacobian(ctx, name, λ):
     """Adjoint Jacobian vs central differences (h = 1e-6 relative) of the oracle residual."""
