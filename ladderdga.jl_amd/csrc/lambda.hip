@@ -227,14 +227,14 @@ __global__ __launch_bounds__(kFT) void lambdadm_final_kernel(OctTables T, FinalA
   }
 }
 
-// G_ladder over fft_range: Sigma_ladder below numax, Sigma_loc above, negative
-// frequencies by conjugation (GFTools.jl:162-186)
 // sigma <- (1 - m) sigma + m sigma_prev (conditions_new.jl:484)
 __global__ void mix_kernel(double2 *__restrict__ sigma, const double2 *__restrict__ prev, double m, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) sigma[i] = cadd(cscale(sigma[i], 1.0 - m), cscale(prev[i], m));
 }
 
+// G_ladder over fft_range: Sigma_ladder below numax, Sigma_loc above, negative
+// frequencies by conjugation (GFTools.jl:162-186)
 __global__ void gladder_kernel(OctTables T, int numax, int fft_first, int n_fft, double beta, const double *res,
                                const double2 *__restrict__ sigma, const double2 *__restrict__ sigloc,
                                double2 *__restrict__ G) {

@@ -305,7 +305,9 @@ def run_sc(χm: χT, γm: γT, χd: χT, γd: γT, λ0, μ: float, Σ_loc, gLoc_
     (:370-372).  "pre_m" first solves λm_correction with λm_rhs (plain mode:
     2 imp_density - Σ χd) and sets λd = 0 (:365-369); the reference's later
     iterations then keep λ fixed for "m" and "pre_m" (its `type == [:pre_m,
-    :m]` test at :450 never holds), so "m" runs as "fix".  The loop stops when both potential energies change by less
+    :m]` test at :456 never holds), so "m" runs as "fix", except that the
+    result of "fix" carries λm = λd = 0 as the reference's does (the λ used
+    are in the trace).  The loop stops when both potential energies change by less
     than conv_abs, or after maxit iterations.  χm and χd are left unchanged.
     """
     import warnings
@@ -329,7 +331,9 @@ def run_sc(χm: χT, γm: γT, χd: χT, γd: γT, λ0, μ: float, Σ_loc, gLoc_
                                             νmax=νmax, λ_min_δ=λ_min_δ, λ_val_only=True, fit_μ=fit_μ,
                                             validate_threshold=conv_abs, tc=tc, ctx=ctx)
     if not validation or not (math.isfinite(λm) and math.isfinite(λd)):
-        return λ_result(λm, λd, type, converged=False, sc_converged=False)
+        nan = math.nan      # conditions_new.jl:391-393: everything NaN when the pre-solve failed
+        return λ_result(nan, nan, type, converged=False, sc_converged=False, EKin=nan, EPot_p1=nan, EPot_p2=nan,
+                        PP_p1=nan, PP_p2=nan, μ=nan, n=nan)
 
     ctx = ctx or default_context()
     ctx.set_kgrid(kG)
@@ -379,7 +383,7 @@ def run_sc(χm: χT, γm: γT, χd: χT, γd: γT, λ0, μ: float, Σ_loc, gLoc_
             E_pot_1_old, E_pot_2_old = E_pot_1, E_pot_2
             E_kin, E_pot_1 = out.Ekin, out.Epot1
             ctx.check(L.ldga_g_tables_dev(ctx.handle, C.byref(fr), d["G"].ptr, d["Gf"].ptr, d["Gr"].ptr, None))
-            if type in ("dm", "pre_dm"):
+            if type in ("m", "pre_m", "dm", "pre_dm"):
                 lhs_c1, E_pot_2 = out.lhs_c1, out.Epot2
             if trace:
                 rows.append(dict(it=it, λm=λm, λd=λd, μ=μ, n=n, EKin=E_kin, EPot=E_pot_1, lhs_c1=lhs_c1,
@@ -398,6 +402,10 @@ def run_sc(χm: χT, γm: γT, χd: χT, γd: γT, λ0, μ: float, Σ_loc, gLoc_
           and abs(E_pot_1 - E_pot_2) <= conv_abs)
     # the reference's constructor stores its 4th/5th arguments swapped (conditions_new.jl:42):
     # `converged` is the sum-rule validation, `sc_converged` the loop's convergence
+    if type == "fix":
+        # run_sc! returns its loop-local λ, which stay 0 for :fix (the χ carry the λ there,
+        # conditions_new.jl:424-425, 461); the trace rows log the λ used
+        λm, λd = 0.0, 0.0
     return λ_result(λm, λd, type, converged=ok, sc_converged=converged, EKin=E_kin, EPot_p1=E_pot_1,
                     EPot_p2=E_pot_2, PP_p1=rhs_c1, PP_p2=lhs_c1, trace=rows, G_ladder=G_ladder, Σ_ladder=Σ_ladder,
                     μ=μ, n=n)

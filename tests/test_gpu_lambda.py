@@ -127,6 +127,9 @@ def test_run_sc_fix(ctx, at, mixing, maxit):
     assert rel(r.Σ_ladder, ref["Σ"]) <= 1e-9 and rel(r.G_ladder, ref["G"]) <= 1e-9
     # χ is left untouched
     assert np.array_equal(p["χm"].data, np.asfortranarray(o["χm"].data))
+    # as the reference's run_sc!, :fix reports its loop-local λ = 0 and logs the λ used in the trace
+    assert (r.λm, r.λd) == (0.0, 0.0)
+    assert all((row["λm"], row["λd"]) == tuple(λ) for row in r.trace)
 
 
 def test_run_sc_pre_dm(ctx):
