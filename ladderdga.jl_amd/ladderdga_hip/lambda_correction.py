@@ -162,13 +162,13 @@ def λm_rhs(imp_density: float, χm: χT, χd: χT, kG: KGrid, mP: ModelParamete
     """
     if χd.λ != 0 and not math.isnan(λd):
         raise RuntimeError(f"λd = {λd} given while χd.λ = {χd.λ} is set")
+    if λ_rhs not in ("native", "fixed"):
+        raise ValueError(f"λ_rhs {λ_rhs!r}: supported are 'native' and 'fixed'")
+    if λ_rhs == "native" and not math.isfinite(imp_density):
+        raise ValueError("imp_density argument is not finite! Cannot use DMFT error compensation method")
     χd_sum = sum_kω(kG, χd, λ=λd, ctx=ctx)
     if λ_rhs == "fixed":
         return mP.n * (1 - mP.n / 2) - χd_sum
-    if λ_rhs != "native":
-        raise ValueError(f"λ_rhs {λ_rhs!r}: supported are 'native' and 'fixed'")
-    if not math.isfinite(imp_density):
-        raise ValueError("imp_density argument is not finite! Cannot use DMFT error compensation method")
     return 2 * imp_density - χd_sum
 
 

@@ -1,0 +1,54 @@
+"""Host-side argument checks of the Python mirror that raise before any GPU work, with
+the reference's messages where it has them (CPU only)."""
+import math
+
+import numpy as np
+import pytest
+
+
+def _χ(Nq=3, Nω=5, **kw):
+    import ladderdga_hip as L
+    return L.χT(np.asfortranarray(np.full((Nq, Nω), 0.2)), 10.0, tail_c=[0, 0, 0.5], **kw)
+
+
+def test_par_api_needs_setup():
+    """calc_Σ_par before initialize_EoM (ladderDGATools.jl:169-171) and calc_bubble_par without the
+    G tables in the worker cache (ladderDGATools.jl:48) raise."""
+    import ladderdga_hip as L
+    L.clear_wcache()
+    with pytest.raises(RuntimeError, match="Call initialize_EoM before calc_Σ_par"):
+        L.calc_Σ_par()
+    with pytest.raises(RuntimeError, match="Call initialize_EoM"):
+        L.calc_Σ_par_inplace(np.zeros((3, 2), complex))
+    with pytest.raises(RuntimeError, match="run lDGA_setup"):
+        L.calc_bubble_par(None, None, None)
+    with pytest.raises(ValueError, match="unsupported field"):
+        L.update_wcache("Σ_loc", np.zeros(3))
+    with pytest.raises(ValueError, match="νGrid"):
+        L.initialize_EoM(None, 0.0, None, range(1, 4), None, None, type("sP", (), {"n_iν": 4})())
+
+
+def test_run_sc_argument_checks():
+    """run_sc rejects unknown types, the χ tail self-consistency and λ-corrected χ
+    (conditions_new.jl:342-380)."""
+    import ladderdga_hip as L
+    χ = _χ()
+    args = (χ, None, χ, None, None, 0.0, None, None, 0.0, None, None, None)
+    with pytest.raises(ValueError, match="run_sc type"):
+        L.run_sc(*args, type="sc")
+    with pytest.raises(NotImplementedError, match="update_χ_tail"):
+        L.run_sc(*args, update_χ_tail=True)
+    with pytest.raises(RuntimeError, match="already set"):
+        L.run_sc(_χ(λ=0.3), None, χ, None, None, 0.0, None, None, 0.0, None, None, None)
+
+
+def test_λm_rhs_argument_checks():
+    """λm_rhs (LambdaCorrection/helpers.jl:234-255): λd with an already corrected χd, an unknown rhs
+    type, and the native rhs without a finite impurity density raise."""
+    import ladderdga_hip as L
+    with pytest.raises(RuntimeError, match="λd"):
+        L.λm_rhs(0.1, _χ(), _χ(λ=0.5), None, None, None, λd=0.2)
+    with pytest.raises(ValueError, match="λ_rhs"):
+        L.λm_rhs(0.1, _χ(), _χ(), None, None, None, λ_rhs="dmft")
+    with pytest.raises(ValueError, match="imp_density"):
+        L.λm_rhs(math.nan, _χ(), _χ(), None, None, None)
