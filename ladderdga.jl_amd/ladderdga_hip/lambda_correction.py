@@ -188,7 +188,8 @@ class LambdaDmProblem:
     """The Î»dm residual with the ladder arrays resident on the device."""
 
     def __init__(self, Ï‡m: Ï‡T, Î³m: Î³T, Ï‡d: Ï‡T, Î³d: Î³T, Î£_loc, gLoc_rfft, Ï‡loc_m_sum, Î»0, kG: KGrid,
-                 mP: ModelParameters, sP: SimulationParameters, Î½max: int, tc: bool = True, ctx=None):
+                 mP: ModelParameters, sP: SimulationParameters, Î½max: int, tc: bool = True, Î¼: float | None = None,
+                 ctx=None):
         self.ctx = ctx or default_context()
         self.ctx.set_kgrid(kG)
         self.kG, self.mP, self.sP, self.Î½max, self.tc = kG, mP, sP, Î½max, tc
@@ -199,7 +200,7 @@ class LambdaDmProblem:
                       Î»0=up(np.asarray(Î»0, np.complex128)), G=up(np.asarray(gLoc_rfft, np.complex128)),
                       Î£loc=up(np.ascontiguousarray(Î£_loc[:sP.fft_range.stop], np.complex128)))
         self.fr, self.mo = _freq(sP), _model(mP)
-        self.Î¼ = mP.Î¼
+        self.Î¼ = mP.Î¼ if Î¼ is None else float(Î¼)      # start of the first Î¼ fit
         self.evals = 0
 
     def eval(self, Î», want_jac=False, fit_Î¼=True, arrays=False):
@@ -236,8 +237,10 @@ class LambdaDmProblem:
 def Î»dm_correction(Ï‡m: Ï‡T, Î³m: Î³T, Ï‡d: Ï‡T, Î³d: Î³T, Î£_loc, gLoc_rfft, Ï‡loc_m_sum, Î»0, kG: KGrid,
                    mP: ModelParameters, sP: SimulationParameters, Î½max: int = -1, Î»_min_Î´: float = 1e-4,
                    Î»_val_only: bool = False, fit_Î¼: bool = True, Î»init=(0.0, 0.0), validate_threshold: float = 1e-8,
-                   tc: bool = True, jacobian: str = "fd", verbose: bool = False, ctx=None):
+                   tc: bool = True, jacobian: str = "fd", verbose: bool = False, Î¼: float | None = None, ctx=None):
     """Î»m, Î»d such that the Pauli sum rule and E_pot consistency hold.
+
+    Î¼ (default mP.Î¼) starts the first Î¼ fit; later fits start from the last one.
 
     jacobian = "fd" reproduces the reference (forward finite differences of the
     residual, helpers.jl:177-180); "adjoint" uses the analytic Jacobian.
@@ -246,7 +249,7 @@ def Î»dm_correction(Ï‡m: Ï‡T, Î³m: Î³T, Ï‡d: Ï‡T, Î³d: Î³T, Î£_loc, gLoc_rfft, Ï
         raise RuntimeError("Î» parameter already set. Aborting Î»dm calculation")
     _, Î½max_d = gen_Î½Ï‰_indices(Ï‡m, Ï‡d, sP)
     Î½max = Î½max_d if Î½max < 1 else min(Î½max_d, Î½max)
-    prob = LambdaDmProblem(Ï‡m, Î³m, Ï‡d, Î³d, Î£_loc, gLoc_rfft, Ï‡loc_m_sum, Î»0, kG, mP, sP, Î½max, tc=tc, ctx=ctx)
+    prob = LambdaDmProblem(Ï‡m, Î³m, Ï‡d, Î³d, Î£_loc, gLoc_rfft, Ï‡loc_m_sum, Î»0, kG, mP, sP, Î½max, tc=tc, Î¼=Î¼, ctx=ctx)
     rhs_c1 = mP.n / 2 * (1 - mP.n / 2)
     f = lambda Î»: prob.eval(Î», fit_Î¼=fit_Î¼)["residual"]
 
@@ -329,7 +332,7 @@ def run_sc(Ï‡m: Ï‡T, Î³m: Î³T, Ï‡d: Ï‡T, Î³d: Î³T, Î»0, Î¼: float, Î£_loc, gLoc_
     if type == "pre_dm":
         Î»m, Î»d, validation = Î»dm_correction(Ï‡m, Î³m, Ï‡d, Î³d, Î£_loc, gLoc_rfft, Ï‡loc_m_sum, Î»0, kG, mP, sP,
                                             Î½max=Î½max, Î»_min_Î´=Î»_min_Î´, Î»_val_only=True, fit_Î¼=fit_Î¼,
-                                            validate_threshold=conv_abs, tc=tc, ctx=ctx)
+                                            validate_threshold=conv_abs, tc=tc, Î¼=Î¼, ctx=ctx)
     if not validation or not (math.isfinite(Î»m) and math.isfinite(Î»d)):
         nan = math.nan      # conditions_new.jl:391-393: everything NaN when the pre-solve failed
         return Î»_result(nan, nan, type, converged=False, sc_converged=False, EKin=nan, EPot_p1=nan, EPot_p2=nan,
@@ -362,7 +365,7 @@ def run_sc(Ï‡m: Ï‡T, Î³m: Î³T, Ï‡d: Ï‡T, Î³d: Î³T, Î»0, Î¼: float, Î£_loc, gLoc_
         while True:
             if type in ("dm", "pre_dm") and it > 1:
                 r = Î»dm_correction(Ï‡m, Î³m, Ï‡d, Î³d, Î£_loc, d["Gr"].to_host(), Ï‡loc_m_sum, Î»0, kG, mP, sP,
-                                   Î½max=Î½max, Î»_min_Î´=Î»_min_Î´, fit_Î¼=fit_Î¼, Î»init=(Î»m, Î»d), tc=tc, ctx=ctx)
+                                   Î½max=Î½max, Î»_min_Î´=Î»_min_Î´, fit_Î¼=fit_Î¼, Î»init=(Î»m, Î»d), tc=tc, Î¼=Î¼, ctx=ctx)
                 Î»m, Î»d, validation = r.Î»m, r.Î»d, r.converged
                 if math.isfinite(r.Î¼):
                     Î¼ = r.Î¼

@@ -831,21 +831,79 @@ def λm_correction(χm: ChiT, rhs, kG, mP, sP):
 
 
 def λdm_correction(χm, γm, χd, γd, Σ_loc, g_rfft, χloc_m_sum, λ0, kG, mP, sP,
-                   νmax=-1, λ_min_δ=1e-4, tc=True, jac=None):
-    """LambdaCorrection/conditions_new.jl:219-324 (lambda values only)."""
+                   νmax=-1, λ_min_δ=1e-4, tc=True, jac=None, λinit=(0.0, 0.0), μ0=None):
+    """LambdaCorrection/conditions_new.jl:219-324 (lambda values only).  Every residual's μ fit
+    starts from μ0 (default mP.mu)."""
+    μ0 = mP.mu if μ0 is None else μ0
     _, νmax_d = gen_νω_indices(χm, χd, mP, sP)
     νmax = νmax_d if νmax < 1 else min(νmax_d, νmax)
     rhs_c1 = mP.n / 2 * (1 - mP.n / 2)
 
     def res(l):
         n, μ, _, ep1, ep2, lhs, _, _ = λdm_residual_vals(l, χm, γm, χd, γd, Σ_loc, g_rfft, χloc_m_sum, λ0,
-                                                         kG, mP, sP, νmax, mP.mu, tc)
+                                                         kG, mP, sP, νmax, μ0, tc)
         return np.array([lhs - rhs_c1, ep1 - ep2])
 
     lm_min = get_λ_min(χm.data)
     ld_min = get_λ_min(χd.data)
     min_λ = np.array([lm_min + λ_min_δ * abs(lm_min), ld_min + λ_min_δ * abs(ld_min)])
-    return newton_right_nd(res, [0.0, 0.0], min_λ, jac=jac), νmax
+    return newton_right_nd(res, list(λinit), min_λ, jac=jac), νmax
+
+
+def run_sc_dm(χm, γm, χd, γd, λ0, μ, Σ_loc, g_rfft, χloc_m_sum, kG, mP, sP, νmax, λm, λd,
+              maxit=100, mixing=0.2, conv_abs=1e-8, tc=True):
+    """run_sc! (LambdaCorrection/conditions_new.jl:405-560) for types :dm / :pre_dm, from (λm, λd).
+
+    The first iteration runs at the given λ (:431-435); every later one first re-solves λdm with the
+    current G_rfft, starting from the last λ and μ (:437-449), and stops the loop when no root
+    validates (:465-471).  After each rebuild the sums are taken at the current λ (:494-499).
+    Returns the run_sc_fix fields plus the last λ.
+    """
+    f0 = sP.fft_range.start
+    rhs_c1 = mP.n / 2 * (1 - mP.n / 2)
+
+    def sums(lm, ld):
+        s_m = sum_kω(kG, ChiT(χ_λ(χm.data, lm), χm.beta, tail_c=χm.tail_c, usable_ω=χm.usable_ω))
+        s_d = sum_kω(kG, ChiT(χ_λ(χd.data, ld), χd.beta, tail_c=χd.tail_c, usable_ω=χd.usable_ω))
+        return (s_d + s_m) / 2, (mP.U / 2) * (s_d - s_m) + mP.U * (mP.n / 2 * mP.n / 2)
+
+    lhs_c1, E_pot_2 = sums(λm, λd)
+    E_pot_1, E_kin = math.inf, math.inf
+    g_rfft = np.array(g_rfft, copy=True)
+    Σ, G, it, converged = None, None, 1, False
+    while True:
+        if it > 1:
+            root, _ = λdm_correction(χm, γm, χd, γd, Σ_loc, g_rfft, χloc_m_sum, λ0, kG, mP, sP, νmax=νmax, tc=tc,
+                                     λinit=(λm, λd), μ0=μ)
+            if not np.all(np.isfinite(root)):
+                λm, λd = root
+                break
+            n_r, μ_r, _, ep1, ep2, lhs, _, _ = λdm_residual_vals(root, χm, γm, χd, γd, Σ_loc, g_rfft, χloc_m_sum,
+                                                                  λ0, kG, mP, sP, νmax, μ, tc)
+            λm, λd = root
+            if not (abs(rhs_c1 - lhs) <= 1e-8 and abs(ep1 - ep2) <= 1e-8):
+                break
+            μ = μ_r
+        Σ_work = Σ
+        Σ = calc_Σ(χm, γm, χd, γd, χloc_m_sum, λ0, g_rfft, kG, mP, sP, νmax=νmax, λm=λm, λd=λd, tc=tc)
+        if mixing != 0 and it > 1:
+            Σ = (1 - mixing) * Σ + mixing * Σ_work
+        μ, G = G_from_Σladder(Σ, Σ_loc, kG, mP, sP, fix_n=True, μ=μ)
+        if math.isnan(μ):
+            break
+        E_pot_1_old, E_pot_2_old = E_pot_1, E_pot_2
+        E_kin, E_pot_1 = calc_E(G, Σ, μ, kG, mP, sP)
+        _, g_rfft = G_fft_from_wedge(G, kG)
+        lhs_c1, E_pot_2 = sums(λm, λd)
+        if abs(E_pot_1 - E_pot_1_old) < conv_abs and abs(E_pot_2 - E_pot_2_old) < conv_abs:
+            converged = True
+            break
+        if it >= maxit:
+            break
+        it += 1
+    n = filling_pos_kG(G[:, -f0:-f0 + νmax], kG, mP.U, μ, mP.beta) if G is not None else math.nan
+    return dict(μ=μ, n=n, E_kin=E_kin, E_pot_1=E_pot_1, E_pot_2=E_pot_2, lhs_c1=lhs_c1, rhs_c1=rhs_c1,
+                sc_converged=converged, it=it, Σ=Σ, G=G, λm=λm, λd=λd)
 
 
 # ----------------------------------------------------------------------------

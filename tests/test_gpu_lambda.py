@@ -156,6 +156,36 @@ def test_run_sc_converges(ctx, mixing, iters):
     assert r.converged == (abs(r.PP_p1 - r.PP_p2) <= 1e-8 and abs(r.EPot_p1 - r.EPot_p2) <= 1e-8)
 
 
+@pytest.mark.parametrize("mixing", [0.0, 0.5])
+def test_run_sc_pre_dm_converges(ctx, mixing):
+    """Full lDΓA_dm self-consistency (run_sc :pre_dm) on the contracting input: λdm is re-solved with
+    every rebuilt G_rfft and the loop converges to a point where both sum rules hold, so the result
+    validates (converged = true).  Against the oracle's loop (O.run_sc_dm) from its λdm root: same
+    fixed point.  Each λ re-solve stops at the Newton tolerance (1e-8 on the residual), so the two
+    loops may stop an iteration apart and agree to that tolerance, not to rounding."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP, o, p, νmax = ladder("conv")
+    print(f"run_sc pre_dm converging mixing={mixing}", flush=True)
+    r = L.run_sc(p["χm"], p["γm"], p["χd"], p["γd"], o["λ0"], inp.mP.μ, inp.Σ_loc, inp.G_rfft, inp.χloc_m_sum,
+                 inp.kG, inp.mP, inp.sP, type="pre_dm", νmax=νmax, maxit=40, mixing=mixing, trace=True, ctx=ctx)
+    root = λdm_root("conv")
+    ref = O.run_sc_dm(o["χm"], o["γm"], o["χd"], o["γd"], o["λ0"], omP.mu, inp.Σ_loc, inp.G_rfft, inp.χloc_m_sum,
+                      oK, omP, osP, νmax, root[0], root[1], maxit=40, mixing=mixing)
+    assert ref["sc_converged"] and r.sc_converged
+    assert abs(len(r.trace) - ref["it"]) <= 2
+    assert np.allclose([r.λm, r.λd], [ref["λm"], ref["λd"]], rtol=1e-6, atol=1e-6)
+    assert np.allclose([r.trace[0]["λm"], r.trace[0]["λd"]], root, rtol=1e-6, atol=1e-6)
+    for a, b in ((r.μ, ref["μ"]), (r.EPot_p1, ref["E_pot_1"]), (r.EPot_p2, ref["E_pot_2"]), (r.PP_p2, ref["lhs_c1"])):
+        assert a == pytest.approx(b, rel=1e-6, abs=1e-7)
+    # the result validates when both sum rules hold at the last iterate (conditions_new.jl:397).  Without
+    # mixing the last iterate is a λdm root (E_pot gap 0 in the oracle); with mixing 0.5 the mixed Σ lags
+    # by the last step (gap 6.5e-9 in the oracle), too close to 1e-8 to require either way
+    assert r.converged == (abs(r.PP_p1 - r.PP_p2) <= 1e-8 and abs(r.EPot_p1 - r.EPot_p2) <= 1e-8)
+    assert abs(r.PP_p1 - r.PP_p2) <= 1e-8
+    if mixing == 0.0:
+        assert r.converged and abs(r.EPot_p1 - r.EPot_p2) <= 1e-9
+
+
 def test_run_sc_pre_dm(ctx):
     """run_sc type :pre_dm: λ from λdm_correction first (the oracle's root), then re-solved with every
     rebuilt G_rfft.  The first iteration runs at the root; on these synthetic inputs a later re-solve
