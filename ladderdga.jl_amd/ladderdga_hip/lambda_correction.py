@@ -412,3 +412,62 @@ def run_sc(χm: χT, γm: γT, χd: χT, γd: γT, λ0, μ: float, Σ_loc, gLoc_
     return λ_result(λm, λd, type, converged=ok, sc_converged=converged, EKin=E_kin, EPot_p1=E_pot_1,
                     EPot_p2=E_pot_2, PP_p1=rhs_c1, PP_p2=lhs_c1, trace=rows, G_ladder=G_ladder, Σ_ladder=Σ_ladder,
                     μ=μ, n=n)
+
+
+# ---------------------------------------------------------------------------
+# the helper and the λ_correction entry point (conditions_new.jl:60-85, 174-190)
+# ---------------------------------------------------------------------------
+@dataclass
+class lDΓAHelper:
+    """The fields of the reference's lDΓAHelper (src/runHelpers.jl:28-49) that the λ corrections read:
+    grid, parameters, local self-energy (ν = 0 .. last(fft_range)), the G_rfft table, the local
+    Σ_ω χm, and the impurity's equal-time same-spin density correlator for the native λm rhs."""
+    kG: KGrid
+    mP: ModelParameters
+    sP: SimulationParameters
+    Σ_loc: np.ndarray
+    gLoc_rfft: np.ndarray
+    χloc_m_sum: float
+    imp_density: float = math.nan
+
+
+def λm_correction_full(χm: χT, γm: γT, χd: χT, γd: γT, λ0, h: lDΓAHelper, νmax: int = -1,
+                       λ_min_δ: float = 1e-4, λ_val_only: bool = False, verbose: bool = False, fit_μ: bool = True,
+                       validate_threshold: float = 1e-8, ctx=None) -> λ_result:
+    """λm from the native rhs, then Σ_ladder at (λm, χd.λ) over ν < νmax (default n_iν), its μ fit,
+    energies and sums (conditions_new.jl:174-190).  As the reference stores it, the result's
+    `converged` is true and `sc_converged` carries the λm validation."""
+    νmax = γm.data.shape[1] // 2 if νmax < 0 else νmax
+    rhs = λm_rhs(h.imp_density, χm, χd, h.kG, h.mP, h.sP, λ_rhs="native", ctx=ctx)
+    λm, validation = λm_correction(χm, rhs, h.kG, h.mP, h.sP, validate_threshold=validate_threshold, ctx=ctx)
+    prob = LambdaDmProblem(χm, γm, χd, γd, h.Σ_loc, h.gLoc_rfft, h.χloc_m_sum, λ0, h.kG, h.mP, h.sP, νmax, ctx=ctx)
+    try:
+        r = prob.eval((λm, χd.λ), fit_μ=fit_μ, arrays=True)
+    finally:
+        prob.free()
+    rhs_c1 = h.mP.n / 2 * (1 - h.mP.n / 2)
+    return λ_result(λm, χd.λ, "m", converged=True, sc_converged=validation, EKin=r["Ekin"], EPot_p1=r["Epot1"],
+                    EPot_p2=r["Epot2"], PP_p1=rhs_c1, PP_p2=r["lhs_c1"], G_ladder=r["G_ladder"],
+                    Σ_ladder=r["Σ_ladder"], μ=r["mu"], n=h.mP.n)
+
+
+def λ_correction(type: str, χm: χT, γm: γT, χd: χT, γd: γT, λ0, h: lDΓAHelper, λm_rhs_type: str = "native",
+                 νmax: int = -1, λ_min_δ: float = 1e-4, maxit: int = 100, mixing: float = 0.2,
+                 conv_abs: float = 1e-8, trace: bool = False, par: bool = False, λ_val_only: bool = False,
+                 verbose: bool = False, validate_threshold: float = 1e-8, tc: bool = True, ctx=None):
+    """The reference's dispatcher (conditions_new.jl:60-85): "m" (λm_correction_full), "dm"
+    (λdm_correction), "sc" (run_sc, type :fix at λ = 0) and "sc_m" (run_sc, type :m).  As there,
+    λm_rhs_type is accepted and not used, and `par` selects nothing on one GPU."""
+    t = type.lstrip(":")
+    if t == "m":
+        return λm_correction_full(χm, γm, χd, γd, λ0, h, νmax=νmax, λ_min_δ=λ_min_δ, verbose=verbose,
+                                  validate_threshold=validate_threshold, ctx=ctx)
+    if t == "dm":
+        return λdm_correction(χm, γm, χd, γd, h.Σ_loc, h.gLoc_rfft, h.χloc_m_sum, λ0, h.kG, h.mP, h.sP,
+                              νmax=νmax, λ_min_δ=λ_min_δ, validate_threshold=validate_threshold, verbose=verbose,
+                              tc=tc, λ_val_only=λ_val_only, ctx=ctx)
+    if t in ("sc", "sc_m"):
+        return run_sc(χm, γm, χd, γd, λ0, h.mP.μ, h.Σ_loc, h.gLoc_rfft, h.χloc_m_sum, h.kG, h.mP, h.sP,
+                      type="fix" if t == "sc" else "m", maxit=maxit, mixing=mixing, conv_abs=conv_abs, trace=trace,
+                      imp_density=h.imp_density, ctx=ctx)
+    raise ValueError(f"λ-correction type '{type}' not recognized!")

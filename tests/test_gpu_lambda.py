@@ -281,3 +281,37 @@ def test_lambdam_correction(ctx):
     got, ok = L.λm_correction(p["χm"], rhs, inp.kG, inp.mP, inp.sP, ctx=ctx)
     assert got == pytest.approx(ref, rel=1e-10)
     assert ok
+
+
+def test_lambda_correction_dispatch(ctx):
+    """λ_correction (conditions_new.jl:60-85) through an lDΓAHelper: "m" is λm_correction_full
+    (:174-190: native rhs, λm root, Σ_ladder over ν < n_iν at (λm, 0), μ fit, energies; the result's
+    flags stored as the reference stores them), "dm" the λdm root, "sc" and "sc_m" the loop at λ = 0."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP, o, p, νmax = ladder("tiny")
+    imp = (O.sum_kω(oK, o["χm"], λ=35.0) + O.sum_kω(oK, o["χd"])) / 2
+    h = L.lDΓAHelper(inp.kG, inp.mP, inp.sP, inp.Σ_loc, inp.G_rfft, inp.χloc_m_sum, imp_density=imp)
+    print("λ_correction m", flush=True)
+    r = L.λ_correction("m", p["χm"], p["γm"], p["χd"], p["γd"], o["λ0"], h, ctx=ctx)
+    λm = O.λm_correction(o["χm"], 2 * imp - O.sum_kω(oK, o["χd"]), oK, omP, osP)
+    assert r.type == "m" and r.λm == pytest.approx(λm, rel=1e-9) and r.λd == 0.0
+    assert r.converged and r.sc_converged            # the λm validation, stored in sc_converged
+    ref = O.λdm_residual_vals(np.array([λm, 0.0]), o["χm"], o["γm"], o["χd"], o["γd"], inp.Σ_loc, inp.G_rfft,
+                              inp.χloc_m_sum, o["λ0"], oK, omP, osP, inp.sP.n_iν, omP.mu)
+    for a, b in ((r.μ, ref[1]), (r.EKin, ref[2]), (r.EPot_p1, ref[3]), (r.EPot_p2, ref[4]), (r.PP_p2, ref[5])):
+        assert a == pytest.approx(b, rel=1e-8, abs=1e-10)
+    assert r.Σ_ladder.shape == (inp.kG.Nq, inp.sP.n_iν) and r.n == inp.mP.n
+    assert np.max(np.abs(r.Σ_ladder - ref[6])) <= 1e-8 * np.max(np.abs(ref[6]))
+    print("λ_correction dm", flush=True)
+    r = L.λ_correction(":dm", p["χm"], p["γm"], p["χd"], p["γd"], o["λ0"], h, ctx=ctx)
+    assert np.allclose([r.λm, r.λd], λdm_root("tiny"), rtol=1e-7, atol=1e-7)
+    print("λ_correction sc, sc_m", flush=True)
+    inp, oK, omP, osP, o, p, νmax = ladder("conv")
+    h = L.lDΓAHelper(inp.kG, inp.mP, inp.sP, inp.Σ_loc, inp.G_rfft, inp.χloc_m_sum)
+    ref = O.run_sc_fix(o["χm"], o["γm"], o["χd"], o["γd"], o["λ0"], omP.mu, inp.Σ_loc, inp.G_rfft, inp.χloc_m_sum,
+                       oK, omP, osP, νmax, 0.0, 0.0, maxit=100, mixing=0.2)
+    for t in ("sc", "sc_m"):
+        r = L.λ_correction(t, p["χm"], p["γm"], p["χd"], p["γd"], o["λ0"], h, ctx=ctx)
+        assert r.sc_converged == ref["sc_converged"] and r.EPot_p1 == pytest.approx(ref["E_pot_1"], rel=1e-9)
+    with pytest.raises(ValueError, match="not recognized"):
+        L.λ_correction("x", p["χm"], p["γm"], p["χd"], p["γd"], o["λ0"], h, ctx=ctx)
