@@ -224,7 +224,9 @@ int ldga_lambdadm_residual_dev(ldga_ctx *ctx, const ldga_freq *fr, const ldga_mo
  * filling_pos(G_ladder[:, 0:numax-1]) = n, G_ladder (Nq x n_fft, nullable) is
  * written, and `out` carries n, mu, E_kin, E_pot1 and, from chi_m / chi_d at
  * (lam_m, lam_d), lhs_c1, E_pot2 and the two residuals (jac is not filled).
- * Sigma_loc is read only for G_ladder.  Synchronises the stream. */
+ * chi_m and chi_d may both be NULL (G_from_Σladder alone): lhs_c1, E_pot2 and
+ * the residuals are then NaN.  Sigma_loc is read only for G_ladder.
+ * Synchronises the stream. */
 int ldga_g_from_sigma_ladder_dev(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m,
                                  double lam_d, int fit_mu, double mu0, const void *chi_m, const void *chi_d,
                                  void *sigma, const void *sigma_prev, double mixing, const void *Sigma_loc,

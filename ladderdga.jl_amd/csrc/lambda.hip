@@ -55,7 +55,8 @@ __global__ __launch_bounds__(kFT) void lambdadm_final_kernel(OctTables T, FinalA
   const double pib = M_PI / beta;
   // ---- χ sums: sum_kω(χ^λ) over all ω and their λ derivatives (DataTypes.jl:230-249)
   double v4[4] = {0.0, 0.0, 0.0, 0.0};
-  const size_t tot = (size_t)A.Nq * A.Nw;
+  const bool have_chi = A.chim != nullptr && A.chid != nullptr;  // NULL: G and energies only
+  const size_t tot = have_chi ? (size_t)A.Nq * A.Nw : 0;
   for (size_t i = tid; i < tot; i += kFT) {
     const int q = (int)(i % A.Nq);
     const double km = T.kmult[q];
@@ -179,8 +180,8 @@ __global__ __launch_bounds__(kFT) void lambdadm_final_kernel(OctTables T, FinalA
   block_sum<6>(v6, red);
   const double nval = filling(mu, v6[0] / Nk);
   const double Ekin = v6[1] / Nk / beta, Epot1 = v6[2] / Nk / beta;
-  const double lhs = 0.5 * (Sd + Sm);
-  const double Epot2 = 0.5 * U * (Sd - Sm) + U * (n / 2.0) * (n / 2.0);
+  const double lhs = have_chi ? 0.5 * (Sd + Sm) : NAN;
+  const double Epot2 = have_chi ? 0.5 * U * (Sd - Sm) + U * (n / 2.0) * (n / 2.0) : NAN;
   if (tid == 0) {
     double *r = A.res;
     r[0] = nval;
@@ -444,7 +445,8 @@ int ldga_g_from_sigma_ladder_dev(ldga_ctx *ctx, const ldga_freq *fr, const ldga_
                                  double lam_d, int fit_mu, double mu0, const void *chi_m, const void *chi_d,
                                  void *sigma, const void *sigma_prev, double mixing, const void *Sigma_loc,
                                  ldga_lambdadm_out *out, void *G_ladder, void *stream) {
-  if (!ctx || !mp || !chi_m || !chi_d || !sigma) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  if (!ctx || !mp || !sigma) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  if (!chi_m != !chi_d) return ldga_fail(ctx, LDGA_ERR_ARG, "chi_m and chi_d are both given or both NULL");
   int rc = ldga_check_freq(ctx, fr);
   if (rc) return rc;
   hipSetDevice(ctx->device);

@@ -471,3 +471,32 @@ def Î»_correction(type: str, Ï‡m: Ï‡T, Î³m: Î³T, Ï‡d: Ï‡T, Î³d: Î³T, Î»0, h: lDÎ
                       type="fix" if t == "sc" else "m", maxit=maxit, mixing=mixing, conv_abs=conv_abs, trace=trace,
                       imp_density=h.imp_density, ctx=ctx)
     raise ValueError(f"Î»-correction type '{type}' not recognized!")
+
+
+def G_from_Î£ladder(Î£_ladder, Î£_loc, kG: KGrid, mP: ModelParameters, sP: SimulationParameters, fix_n: bool = False,
+                   Î¼: float | None = None, ctx=None):
+    """(Î¼, G_ladder) from Î£_ladder (Nq x Î½max, Î½ = 0..Î½max-1) over sP.fft_range (GFTools.jl:154-186):
+    Î£_ladder below Î½max, Î£_loc (Î½ = 0..last(fft_range)) above, negative Î½ by conjugation; with fix_n,
+    Î¼ (starting from Î¼, default mP.Î¼) is refit so that filling_pos(G[:, 0:Î½max-1]) = n.  Î¼ is NaN
+    when the fit fails, as in the reference."""
+    ctx = ctx or default_context()
+    ctx.set_kgrid(kG)
+    Î£_ladder = np.asarray(Î£_ladder, np.complex128)
+    if Î£_ladder.ndim != 2 or Î£_ladder.shape[0] != kG.Nq:
+        raise ValueError(f"Î£_ladder must be Nq x Î½max with Nq = {kG.Nq}")
+    Î½max = Î£_ladder.shape[1]
+    Î¼0 = mP.Î¼ if Î¼ is None else float(Î¼)
+    up = ctx.to_device
+    Î£d = up(Î£_ladder)
+    Î£l = up(np.ascontiguousarray(np.asarray(Î£_loc)[:sP.fft_range.stop], np.complex128))
+    G = ctx.empty((kG.Nq, len(sP.fft_range)), np.complex128)
+    out = LambdaDmOut()
+    try:
+        ctx.check(_lib.lib().ldga_g_from_sigma_ladder_dev(ctx.handle, C.byref(_freq(sP)), C.byref(_model(mP)), Î½max,
+                                                          0.0, 0.0, int(fix_n), Î¼0, None, None, Î£d.ptr, None, 0.0,
+                                                          Î£l.ptr, C.byref(out), G.ptr, None))
+        G_new = G.to_host()
+    finally:
+        for a in (Î£d, Î£l, G):
+            a.free()
+    return (out.mu if fix_n else Î¼0), G_new

@@ -315,3 +315,20 @@ def test_lambda_correction_dispatch(ctx):
         assert r.sc_converged == ref["sc_converged"] and r.EPot_p1 == pytest.approx(ref["E_pot_1"], rel=1e-9)
     with pytest.raises(ValueError, match="not recognized"):
         L.λ_correction("x", p["χm"], p["γm"], p["χd"], p["γd"], o["λ0"], h, ctx=ctx)
+
+
+@pytest.mark.parametrize("fix_n,μ", [(True, None), (False, 0.3), (True, 2.5)])
+def test_G_from_Σladder(ctx, fix_n, μ):
+    """G_from_Σladder (GFTools.jl:154-186) alone, through the entry point without χ: the μ fit and
+    G over fft_range against the oracle, on a ladder Σ of the tiny system."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP, o, p, νmax = ladder("tiny")
+    Σ = oracle_res("tiny", (94.0, -26.0))["Σ"]
+    print(f"G_from_Σladder fix_n={fix_n} μ={μ}", flush=True)
+    μg, G = L.G_from_Σladder(Σ, inp.Σ_loc, inp.kG, inp.mP, inp.sP, fix_n=fix_n, μ=μ, ctx=ctx)
+    μr, Gr = O.G_from_Σladder(Σ, inp.Σ_loc, oK, omP, osP, fix_n=fix_n, μ=μ)
+    assert μg == pytest.approx(μr, rel=1e-10, abs=1e-12)
+    assert np.max(np.abs(G - Gr)) <= 1e-10 * np.max(np.abs(Gr))
+    if fix_n:
+        f0 = osP.fft_range.start
+        assert O.filling_pos_kG(G[:, -f0:-f0 + Σ.shape[1]], oK, omP.U, μg, omP.beta) == pytest.approx(omP.n, abs=1e-10)
