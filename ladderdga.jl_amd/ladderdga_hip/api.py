@@ -180,6 +180,26 @@ def sum_kω(kG: KGrid, χ: χT, λ: float = math.nan, force_full_range=False, ct
     return (out[0], out[2]) if with_derivative else float(out[0])
 
 
+def sum_ωk(kG: KGrid, χ: χT, λ: float = math.nan, force_full_range=False, ctx=None) -> float:
+    """kintegrate(sum_ω(χ)) (src/DataTypes.jl:257-264): the ω sum per q first, then the q sum.  The
+    two sums are finite and linear, so this is sum_kω in the other order; it runs the same GPU
+    reduction."""
+    if χ.λ != 0 and not math.isnan(λ) and λ != 0:
+        raise RuntimeError("χ already λ-corrected, but external λ provided!")
+    return sum_kω(kG, χ, λ=λ, force_full_range=force_full_range, ctx=ctx)
+
+
+def Σ_loc_correction(Σ_ladder, Σ_ladderLoc, Σ_loc):
+    """Σ_ladder[q, ν] - Σ_ladderLoc[0, ν] + Σ_loc[ν] (src/ladderDGATools.jl:249-257), on the host as
+    in the reference: O(Nq νmax) additions on arrays the caller holds."""
+    Σ_ladder = np.asarray(Σ_ladder)
+    Σ_ladderLoc = np.asarray(Σ_ladderLoc)
+    νmax = Σ_ladder.shape[1]
+    if Σ_ladderLoc.ndim != 2 or Σ_ladderLoc.shape[1] < νmax or len(Σ_loc) < νmax:
+        raise ValueError("Σ_ladderLoc (1 x ≥νmax) and Σ_loc (≥νmax) must cover ν = 0..νmax-1")
+    return Σ_ladder - Σ_ladderLoc[0, :νmax][None, :] + np.asarray(Σ_loc)[:νmax][None, :]
+
+
 def get_λ_min(χr: np.ndarray) -> float:
     """-min_q 1/χ(q, ω₀) (LambdaCorrection/helpers.jl:217-220)."""
     nh = math.ceil(χr.shape[1] / 2) - 1

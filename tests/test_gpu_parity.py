@@ -315,6 +315,8 @@ def test_sum_kω(ctx, lam):
     got, deriv = L.sum_kω(inp.kG, χ, λ=lam, ctx=ctx, with_derivative=True)
     ref = O.sum_kω(oK, oχ, λ=lam if lam != 0 else None)
     assert got == pytest.approx(ref, rel=1e-13, abs=1e-14)
+    # sum_ωk (DataTypes.jl:257-264): the oracle sums ω first per q, with sum_ω's tail handling
+    assert L.sum_ωk(inp.kG, χ, λ=lam, ctx=ctx) == pytest.approx(O.sum_ωk(oK, oχ, λ=lam), rel=1e-12, abs=1e-14)
     h = 1e-6
     fd = (O.sum_kω(oK, oχ, λ=lam + h) - O.sum_kω(oK, oχ, λ=lam - h)) / (2 * h)
     assert deriv == pytest.approx(fd, rel=1e-6)

@@ -52,3 +52,17 @@ def test_λm_rhs_argument_checks():
         L.λm_rhs(0.1, _χ(), _χ(), None, None, None, λ_rhs="dmft")
     with pytest.raises(ValueError, match="imp_density"):
         L.λm_rhs(math.nan, _χ(), _χ(), None, None, None)
+
+
+def test_Σ_loc_correction():
+    """ladderDGATools.jl:249-257: Σ_ladder - Σ_ladderLoc[1, ν] + Σ_loc[ν]; with Σ_ladderLoc equal to Σ_loc
+    it is the identity, and a constant shift of Σ_ladderLoc shifts the result by minus that constant."""
+    import ladderdga_hip as L
+    rng = np.random.default_rng(3)
+    Σ = rng.standard_normal((5, 4)) + 1j * rng.standard_normal((5, 4))
+    Σloc = rng.standard_normal(7) + 1j * rng.standard_normal(7)
+    assert np.allclose(L.Σ_loc_correction(Σ, Σloc[None, :], Σloc), Σ, rtol=0, atol=1e-15)
+    out = L.Σ_loc_correction(Σ, Σloc[None, :] + 0.25j, Σloc)
+    assert np.allclose(out, Σ - 0.25j, rtol=0, atol=1e-15)
+    with pytest.raises(ValueError):
+        L.Σ_loc_correction(Σ, Σloc[None, :3], Σloc)
