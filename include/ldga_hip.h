@@ -136,6 +136,17 @@ int ldga_chi_gamma_dev(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr,
                        const void *Gamma, const void *chi0, void *chi, void *gamma, ldga_bse_report *report,
                        void *stream);
 
+/* ---- local full vertex: F_from_χ (src/BSETools.jl:40-53) ---------------
+ * F[nu,nu',w] = -(chi[nu,nu',w] + diag_term delta_{nu nu'} beta G(nu) G(nu+w))
+ *               / (G(nu) G(nu+w) G(nu') G(nu'+w)),
+ * chi and F (Nnu x Nnu x Nw) complex, G (n_fft) the local G over fft_range
+ * (e.g. gImp).  Needs no k grid.  LDGA_ERR_SHAPE when fft_range does not
+ * cover every nu and nu+w the formula reads. */
+int ldga_F_from_chi(ldga_ctx *ctx, const ldga_freq *fr, double beta, int diag_term, const double *chi,
+                    const double *G, double *F);
+int ldga_F_from_chi_dev(ldga_ctx *ctx, const ldga_freq *fr, double beta, int diag_term, const void *chi,
+                        const void *G, void *F, void *stream);
+
 /* ---- lambda_0: calc_λ0 plain branch (src/BSETools.jl:128-141) ----------
  * lambda0[q,nu',w] = chi0[q, shell+nu', w]/beta^2 * sum_nu F[nu,nu',w]. */
 int ldga_lambda0(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double *F, const double *chi0,

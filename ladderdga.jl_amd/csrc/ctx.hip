@@ -419,6 +419,33 @@ int ldga_lambda0_dev(ldga_ctx *ctx, const ldga_freq *fr, double beta, const void
                         ldga_stream(ctx, stream));
 }
 
+int ldga_F_from_chi(ldga_ctx *ctx, const ldga_freq *fr, double beta, int diag_term, const double *chi,
+                    const double *G, double *F) {
+  if (!ctx || !chi || !G || !F) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  int rc = ldga_check_freq(ctx, fr);
+  if (rc) return rc;
+  hipSetDevice(ctx->device);
+  const size_t Nnu = 2 * (size_t)fr->n_inu, Nw = 2 * (size_t)fr->n_iw + 1, bytes = Nnu * Nnu * Nw * 16;
+  void *c, *g, *f;
+  if ((rc = h2d(ctx, "fv_chi", chi, bytes, &c))) return rc;
+  if ((rc = h2d(ctx, "fv_G", G, (size_t)fr->n_fft * 16, &g))) return rc;
+  if (!(f = ldga_buf(ctx, "fv_F", bytes))) return LDGA_ERR_HIP;
+  if ((rc = launch_F_from_chi(ctx, fr, beta, diag_term, (const double2 *)c, (const double2 *)g, (double2 *)f,
+                              ctx->stream)))
+    return rc;
+  return d2h(ctx, F, f, bytes);
+}
+
+int ldga_F_from_chi_dev(ldga_ctx *ctx, const ldga_freq *fr, double beta, int diag_term, const void *chi,
+                        const void *G, void *F, void *stream) {
+  if (!ctx || !chi || !G || !F) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  int rc = ldga_check_freq(ctx, fr);
+  if (rc) return rc;
+  hipSetDevice(ctx->device);
+  return launch_F_from_chi(ctx, fr, beta, diag_term, (const double2 *)chi, (const double2 *)G, (double2 *)F,
+                           ldga_stream(ctx, stream));
+}
+
 int ldga_sum_kw(ldga_ctx *ctx, const double *chi, int Nw, int w_lo, int w_hi, double c2, double beta, double lam,
                 double out[3]) {
   if (!ctx || !chi || !out) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");

@@ -515,6 +515,46 @@ int launch_lambda0(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double
   return LDGA_OK;
 }
 
+// F[nu, nu', w] = -(chi[nu, nu', w] + delta_{nu nu'} beta G(nu) G(nu+w)) / (G(nu) G(nu+w) G(nu') G(nu'+w))
+// (F_from_χ, BSETools.jl:40-53); nu_n of index a at w: a - n_inu - trunc(shift w / 2) (helpers.jl:46-50).
+// One thread per element, nu fastest, as the column-major arrays.
+__global__ void fvertex_kernel(int Nnu, int Nw, int n_iw, int n_inu, int shift, int fft_first, double beta, int diag,
+                               const double2 *__restrict__ chi, const double2 *__restrict__ G,
+                               double2 *__restrict__ F) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)Nnu * Nnu * Nw) return;
+  const int a = (int)(i % Nnu);
+  const size_t r = i / Nnu;
+  const int b = (int)(r % Nnu), wi = (int)(r / Nnu);
+  const int wn = wi - n_iw, off = -n_inu - trunc_half(shift * wn) - fft_first;
+  const double2 g12 = cmul(G[a + off], G[a + wn + off]);
+  const double2 g34 = cmul(G[b + off], G[b + wn + off]);
+  double2 num = chi[i];
+  if (diag && a == b) num = cadd(num, cscale(g12, beta));
+  F[i] = cscale(cdiv(num, cmul(g12, g34)), -1.0);
+}
+
+int launch_F_from_chi(ldga_ctx *ctx, const ldga_freq *fr, double beta, int diag_term, const double2 *chi,
+                      const double2 *G, double2 *F, hipStream_t st) {
+  const int Nnu = 2 * fr->n_inu, Nw = 2 * fr->n_iw + 1;
+  // every G index the kernel forms must lie in fft_range
+  int lo = 0, hi = 0;
+  for (int wi = 0; wi < Nw; ++wi) {
+    const int wn = wi - fr->n_iw, first = -fr->n_inu - (fr->shift * wn) / 2, last = first + Nnu - 1;
+    const int l = std::min(first, first + wn), h = std::max(last, last + wn);
+    if (wi == 0 || l < lo) lo = l;
+    if (wi == 0 || h > hi) hi = h;
+  }
+  if (lo < fr->fft_first || hi > fr->fft_first + fr->n_fft - 1)
+    return ldga_fail(ctx, LDGA_ERR_SHAPE, "G over fft_range [%d,%d] does not cover nu in [%d,%d]", fr->fft_first,
+                     fr->fft_first + fr->n_fft - 1, lo, hi);
+  const size_t total = (size_t)Nnu * Nnu * Nw;
+  fvertex_kernel<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(Nnu, Nw, fr->n_iw, fr->n_inu, fr->shift,
+                                                                  fr->fft_first, beta, diag_term, chi, G, F);
+  LDGA_HIP_TRY(ctx, hipGetLastError());
+  return LDGA_OK;
+}
+
 int launch_sum_kw(ldga_ctx *ctx, const double *chi, int Nw, int w_lo, int w_hi, double c2, double beta, double lam,
                   double *out_dev, hipStream_t st) {
   if (Nw < 1 || (Nw & 1) == 0 || w_lo < 0 || w_hi >= Nw || w_lo > w_hi)

@@ -104,6 +104,25 @@ def calc_χγ(type_: str, Γr, χ0: χ0T, kG: KGrid, mP: ModelParameters, sP: Si
 
 
 # ---------------------------------------------------------------------------
+# local full vertex (src/BSETools.jl:40-53)
+# ---------------------------------------------------------------------------
+def F_from_χ(χ, G, sP: SimulationParameters, β: float, diag_term: bool = True, ctx=None):
+    """F[ν,ν',ω] = -(χ[ν,ν',ω] + δ_νν' β G(ν)G(ν+ω)) / (G(ν)G(ν+ω)G(ν')G(ν'+ω)) from the local χ
+    (2n_iν x 2n_iν x 2n_iω+1) and the local G over sP.fft_range (e.g. gImp)."""
+    ctx = ctx or default_context()
+    Nν, Nω = 2 * sP.n_iν, 2 * sP.n_iω + 1
+    if χ.shape != (Nν, Nν, Nω):
+        raise ValueError(f"χ must be {Nν}x{Nν}x{Nω}")
+    G = np.ascontiguousarray(G, dtype=c128)
+    if G.shape != (len(sP.fft_range),):
+        raise ValueError(f"G must hold the {len(sP.fft_range)} frequencies of fft_range")
+    out = np.empty((Nν, Nν, Nω), dtype=c128, order="F")
+    ctx.check(_lib.lib().ldga_F_from_chi(ctx.handle, C.byref(_freq(sP)), β, int(diag_term), ptr(χ, c128, "χ"),
+                                         G.ctypes.data, out.ctypes.data))
+    return out
+
+
+# ---------------------------------------------------------------------------
 # λ0, plain sums (src/BSETools.jl:118-144)
 # ---------------------------------------------------------------------------
 def calc_λ0(χ0: χ0T, Fr, mP: ModelParameters, sP: SimulationParameters, kG: KGrid, ctx=None):

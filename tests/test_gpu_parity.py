@@ -451,3 +451,23 @@ def test_sigma_large(ctx):
     ref = O.calc_Σ(O.ChiT(cm, omP.beta, tail_c=[0, 0, c2]), gm, O.ChiT(cd, omP.beta, tail_c=[0, 0, c2]), gd, 0.2,
                    l0, inp.G_rfft, oK, omP, osP, νmax=45, λm=0.1, λd=0.05)
     assert relerr(Σ, ref) <= TOL
+
+
+@pytest.mark.parametrize("kw", [dict(n_iω=3, n_iν=4, shift=1), dict(n_iω=5, n_iν=6, shift=0), dict(n_iω=4, n_iν=3, shift=1)])
+@pytest.mark.parametrize("diag", [True, False])
+def test_F_from_χ(ctx, kw, diag):
+    """F_from_χ (BSETools.jl:40-53) on the GPU against the oracle, random local χ and G over fft_range."""
+    import ladderdga_hip as L
+    from ladderdga_hip.types import SimulationParameters
+    print(f"F_from_χ {kw} diag={diag}", flush=True)
+    osP = O.SimulationParameters(kw["n_iω"], kw["n_iν"], 0, kw["shift"])
+    sP = SimulationParameters(kw["n_iω"], kw["n_iν"], 0, kw["shift"])
+    assert sP.fft_range == osP.fft_range
+    rng = np.random.default_rng(21)
+    Nν, Nω, nf = 2 * kw["n_iν"], 2 * kw["n_iω"] + 1, len(osP.fft_range)
+    χ = np.asfortranarray(rng.standard_normal((Nν, Nν, Nω)) + 1j * rng.standard_normal((Nν, Nν, Nω)))
+    G = 1.0 / (1j * (2 * np.arange(osP.fft_range.start, osP.fft_range.stop) + 1) * np.pi / 4.0 + 0.3
+               - 0.2 * rng.standard_normal(nf))
+    F = L.F_from_χ(χ, G, sP, 4.0, diag_term=diag, ctx=ctx)
+    ref = O.F_from_χ(χ, (G, osP.fft_range.start), osP, 4.0, diag_term=diag)
+    assert relerr(F, ref) <= 1e-13

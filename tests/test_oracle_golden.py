@@ -314,3 +314,28 @@ def test_calc_E_limits():
     assert E_kin == pytest.approx(float(kG.kintegrate(2 * kG.eps_wedge * fermi)), abs=2e-5)
     assert E_pot == pytest.approx(0.0, abs=1e-12)
     del nf
+
+
+@pytest.mark.parametrize("shift", [0, 1])
+def test_F_from_χ_closed_forms(shift):
+    """F_from_χ (BSETools.jl:40-53) on χ built from the four G factors it divides by: the local bubble
+    χ = -β G(ν)G(ν+ω) δ_νν' gives F = 0, and χ = c G(ν)G(ν+ω)G(ν')G(ν'+ω) gives F = -c (diagonal term
+    off).  A wrong ν index, ω offset or shift fails both."""
+    sP = O.SimulationParameters(3, 4, 0, shift)
+    rng = np.random.default_rng(11 + shift)
+    first = sP.fft_range.start
+    Gf = rng.standard_normal(len(sP.fft_range)) + 1j * rng.standard_normal(len(sP.fft_range))
+    Nν, Nω = 2 * sP.n_inu, 2 * sP.n_iw + 1
+    χ0 = np.zeros((Nν, Nν, Nω), complex)
+    χc = np.zeros((Nν, Nν, Nω), complex)
+    β, c = 3.0, 0.7 - 0.2j
+    for ωi in range(1, Nω + 1):
+        for νi in range(1, Nν + 1):
+            ωn, νn = O.OneToIndex_to_Freq(ωi, νi, sP)
+            χ0[νi - 1, νi - 1, ωi - 1] = -β * Gf[νn - first] * Gf[ωn + νn - first]
+            for νpi in range(1, Nν + 1):
+                _, νpn = O.OneToIndex_to_Freq(ωi, νpi, sP)
+                χc[νi - 1, νpi - 1, ωi - 1] = c * (Gf[νn - first] * Gf[ωn + νn - first] * Gf[νpn - first]
+                                                   * Gf[ωn + νpn - first])
+    assert np.max(np.abs(O.F_from_χ(χ0, (Gf, first), sP, β))) <= 1e-12
+    assert np.allclose(O.F_from_χ(χc, (Gf, first), sP, β, diag_term=False), -c, rtol=1e-12, atol=1e-12)
