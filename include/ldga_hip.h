@@ -136,6 +136,15 @@ int ldga_chi_gamma_dev(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr,
                        const void *Gamma, const void *chi0, void *chi, void *gamma, ldga_bse_report *report,
                        void *stream);
 
+/* ---- generalized susceptibility: calc_gen_χ (src/BSETools.jl:259-275) --
+ * inv[:,:,q,w] = (Gamma[:,:,w] + diag(1/chi0[q, shell+1:shell+Nnu, w]))^-1,
+ * inv (Nnu x Nnu x Nq x Nw) complex, in FP64 by Gauss-Jordan with column
+ * pivoting in global memory (meant for analysis, not the hot path).
+ * Synchronises the stream; LDGA_ERR_ZERO_PIVOT when a matrix is exactly
+ * singular (its inverse then holds NaN/Inf). */
+int ldga_gen_chi(ldga_ctx *ctx, const ldga_freq *fr, const double *Gamma, const double *chi0, double *inv);
+int ldga_gen_chi_dev(ldga_ctx *ctx, const ldga_freq *fr, const void *Gamma, const void *chi0, void *inv, void *stream);
+
 /* ---- local full vertex: F_from_χ (src/BSETools.jl:40-53) ---------------
  * F[nu,nu',w] = -(chi[nu,nu',w] + diag_term delta_{nu nu'} beta G(nu) G(nu+w))
  *               / (G(nu) G(nu+w) G(nu') G(nu'+w)),

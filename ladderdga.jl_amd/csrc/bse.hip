@@ -275,7 +275,8 @@ int launch_bse_blk_f64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, 
                        unsigned long long *counters, const int *only, int n_only, hipStream_t st);
 int launch_bse_gmem_f64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam,
                         const double2 *chi0, double s, double U, double beta, double *chi, double2 *gamma,
-                        unsigned long long *cnt, const int *only, int n_only, hipStream_t st);
+                        unsigned long long *cnt, const int *only, int n_only, hipStream_t st,
+                        double2 *inv);
 
 // FP64 over every matrix or the listed ones: the blocked MFMA kernel up to
 // Nnu = 128, the global-memory kernel above.
@@ -284,7 +285,8 @@ static int launch_fp64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, 
                        unsigned long long *cnt, const int *only, int n_only, hipStream_t st) {
   if (N <= 128)
     return launch_bse_blk_f64(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, chi, gamma, cnt, only, n_only, st);
-  return launch_bse_gmem_f64(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, chi, gamma, cnt, only, n_only, st);
+  return launch_bse_gmem_f64(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, chi, gamma, cnt, only, n_only, st,
+                             nullptr);
 }
 
 // Clears the report counters with a kernel rather than hipMemsetAsync, so that
@@ -295,6 +297,27 @@ __global__ void bse_clear_counters(unsigned long long *cnt, int n) {
   if ((int)threadIdx.x < n) cnt[threadIdx.x] = 0ull;
 }
 }  // namespace
+
+// calc_gen_χ (BSETools.jl:259-275): the full inverses of Gamma[:,:,w] +
+// diag(1/chi0[q, shell+1:shell+Nnu, w]) by the global-memory FP64 kernel into
+// inv (Nnu x Nnu x Nq x Nw).  Synchronises the stream to read the zero-pivot
+// count (LDGA_ERR_ZERO_PIVOT: those inverses hold NaN/Inf, as inv would).
+int launch_gen_chi(ldga_ctx *ctx, const ldga_freq *fr, const double2 *Gam, const double2 *chi0, double2 *inv,
+                   hipStream_t st) {
+  const int N = 2 * fr->n_inu, Nw = 2 * fr->n_iw + 1, Nq = ctx->Nq, Nc = N + 2 * fr->n_inu_shell;
+  unsigned long long *cnt = (unsigned long long *)ldga_buf(ctx, "gen_chi_counters", 8 * 8);
+  if (!cnt) return LDGA_ERR_HIP;
+  bse_clear_counters<<<1, 8, 0, st>>>(cnt, 8);
+  LDGA_HIP_TRY(ctx, hipGetLastError());
+  int rc = launch_bse_gmem_f64(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, 1.0, 0.0, 1.0, nullptr, nullptr, cnt,
+                               nullptr, 0, st, inv);
+  if (rc) return rc;
+  unsigned long long zeros = 0;
+  LDGA_HIP_TRY(ctx, hipMemcpyAsync(&zeros, cnt, 8, hipMemcpyDeviceToHost, st));
+  LDGA_HIP_TRY(ctx, hipStreamSynchronize(st));
+  if (zeros) return ldga_fail(ctx, LDGA_ERR_ZERO_PIVOT, "%llu matrices with an exactly zero pivot", zeros);
+  return LDGA_OK;
+}
 
 // The register-tile Gauss-Jordan kernel above (Nnu <= 127) is kept as the
 // second FP64 implementation; LDGA_BSE_F64=gj selects it instead of the

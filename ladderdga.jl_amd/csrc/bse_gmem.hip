@@ -29,7 +29,8 @@ __global__ __launch_bounds__(GM_THREADS) void bse_gmem_f64_kernel(int N, int Nq,
                                                                   double2 *__restrict__ gamma_out,
                                                                   unsigned long long *__restrict__ counters,
                                                                   const int *__restrict__ only, int n_items,
-                                                                  double2 *__restrict__ ws) {
+                                                                  double2 *__restrict__ ws,
+                                                                  double2 *__restrict__ inv_out) {
   extern __shared__ double2 lds[];
   double2 *rs = lds;         // pivot row / piv
   double2 *fcol = rs + N;    // pivot column before the step
@@ -133,6 +134,15 @@ __global__ __launch_bounds__(GM_THREADS) void bse_gmem_f64_kernel(int N, int Nq,
       __syncthreads();
     }
 
+    // inv_out: the full inverse, A^-1[i][p_r] = M[r][p_i] (the matrix is M = A^T), stored
+    // column-major per (q, w) as calc_gen_χ returns it (BSETools.jl:259-275)
+    if (inv_out) {
+      double2 *X = inv_out + (size_t)(q + (size_t)Nq * w) * N * N;
+      for (int k = tid; k < N * N; k += GM_THREADS) {
+        const int i = k % N, r = k / N;
+        X[i + (size_t)N * pcol[r]] = M[(size_t)N * r + pcol[i]];
+      }
+    }
     // y[p_r] = sum_c M[r][c]; chi = Re sum y / beta^2; gamma = y / (chi0 (1 + s U chi))
     for (int r = tid; r < N; r += GM_THREADS) {
       double2 acc = cz(0.0, 0.0);
@@ -149,12 +159,12 @@ __global__ __launch_bounds__(GM_THREADS) void bse_gmem_f64_kernel(int N, int Nq,
     for (int k = 0; k < GM_THREADS / 64; ++k) tot += red_chi[k];
     const double chi = tot / (beta * beta);
     const double fac = 1.0 + s * U * chi;
-    for (int k = tid; k < N; k += GM_THREADS) {
+    for (int k = tid; gamma_out && k < N; k += GM_THREADS) {
       const double2 c0v = chi0[q + (size_t)Nq * (k + (size_t)Nc * w)];  // no shell offset (BSETools.jl:236)
       gamma_out[q + (size_t)Nq * (k + (size_t)N * w)] = cdiv(y[k], cscale(c0v, fac));
     }
     if (tid == 0) {
-      chi_out[q + (size_t)Nq * w] = chi;
+      if (chi_out) chi_out[q + (size_t)Nq * w] = chi;
       if (zero) atomicAdd(&counters[0], 1ull);
     }
     __syncthreads();  // LDS and the workspace slot are reused by the next matrix
@@ -164,10 +174,11 @@ __global__ __launch_bounds__(GM_THREADS) void bse_gmem_f64_kernel(int N, int Nq,
 }  // namespace
 
 // FP64 Gauss-Jordan in global memory: every (q, w), or the n_only matrices
-// (q + Nq w) listed in only.  Any Nnu up to 1024.
+// (q + Nq w) listed in only.  Any Nnu up to 1024.  chi and gamma may be NULL;
+// inv (NULL or Nnu x Nnu x Nq x Nw) receives the full inverses.
 int launch_bse_gmem_f64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam,
                         const double2 *chi0, double s, double U, double beta, double *chi, double2 *gamma,
-                        unsigned long long *cnt, const int *only, int n_only, hipStream_t st) {
+                        unsigned long long *cnt, const int *only, int n_only, hipStream_t st, double2 *inv) {
   if (N < 1 || N > 1024) return ldga_fail(ctx, LDGA_ERR_SHAPE, "FP64 global-memory BSE supports Nnu <= 1024 (got %d)", N);
   const int n_items = only ? n_only : Nq * Nw;
   if (n_items == 0) return LDGA_OK;
@@ -179,7 +190,7 @@ int launch_bse_gmem_f64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell,
   if (!ws) return LDGA_ERR_HIP;
   const size_t lds = (size_t)N * (3 * sizeof(double2) + 2 * sizeof(int));
   bse_gmem_f64_kernel<<<dim3((unsigned)grid), dim3(GM_THREADS), lds, st>>>(N, Nq, Nc, shell, Gam, chi0, s, U, beta,
-                                                                          chi, gamma, cnt, only, n_items, ws);
+                                                                          chi, gamma, cnt, only, n_items, ws, inv);
   LDGA_HIP_TRY(ctx, hipGetLastError());
   return LDGA_OK;
 }

@@ -419,6 +419,34 @@ int ldga_lambda0_dev(ldga_ctx *ctx, const ldga_freq *fr, double beta, const void
                         ldga_stream(ctx, stream));
 }
 
+int ldga_gen_chi(ldga_ctx *ctx, const ldga_freq *fr, const double *Gamma, const double *chi0, double *inv) {
+  if (!ctx || !Gamma || !chi0 || !inv) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  NEED_KGRID(ctx);
+  int rc = ldga_check_freq(ctx, fr);
+  if (rc) return rc;
+  hipSetDevice(ctx->device);
+  const size_t Nnu = 2 * (size_t)fr->n_inu, Nw = 2 * (size_t)fr->n_iw + 1, Nq = ctx->Nq;
+  const size_t gb = Nnu * Nnu * Nw * 16, cb = Nq * (Nnu + 2 * fr->n_inu_shell) * Nw * 16, ob = Nnu * Nnu * Nq * Nw * 16;
+  void *g, *c, *o;
+  if ((rc = h2d(ctx, "Gamma", Gamma, gb, &g))) return rc;
+  if ((rc = h2d(ctx, "chi0", chi0, cb, &c))) return rc;
+  if (!(o = ldga_buf(ctx, "gen_chi", ob))) return LDGA_ERR_HIP;
+  const int st = launch_gen_chi(ctx, fr, (const double2 *)g, (const double2 *)c, (double2 *)o, ctx->stream);
+  if (st && st != LDGA_ERR_ZERO_PIVOT) return st;
+  if ((rc = d2h(ctx, inv, o, ob))) return rc;
+  return st;
+}
+
+int ldga_gen_chi_dev(ldga_ctx *ctx, const ldga_freq *fr, const void *Gamma, const void *chi0, void *inv, void *stream) {
+  if (!ctx || !Gamma || !chi0 || !inv) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  NEED_KGRID(ctx);
+  int rc = ldga_check_freq(ctx, fr);
+  if (rc) return rc;
+  hipSetDevice(ctx->device);
+  return launch_gen_chi(ctx, fr, (const double2 *)Gamma, (const double2 *)chi0, (double2 *)inv,
+                        ldga_stream(ctx, stream));
+}
+
 int ldga_F_from_chi(ldga_ctx *ctx, const ldga_freq *fr, double beta, int diag_term, const double *chi,
                     const double *G, double *F) {
   if (!ctx || !chi || !G || !F) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");

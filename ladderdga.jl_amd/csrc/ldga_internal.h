@@ -104,6 +104,8 @@ int launch_sum_kw(ldga_ctx *ctx, const double *chi, int Nw, int w_lo, int w_hi, 
                   double *out_dev, hipStream_t st);
 int launch_F_from_chi(ldga_ctx *ctx, const ldga_freq *fr, double beta, int diag_term, const double2 *chi,
                       const double2 *G, double2 *F, hipStream_t st);
+int launch_gen_chi(ldga_ctx *ctx, const ldga_freq *fr, const double2 *Gam, const double2 *chi0, double2 *inv,
+                   hipStream_t st);
 // EoM partial sums: one (nu, omega-chunk) workgroup each; returns the chunking used
 int eom_chunks(int Nw, int numax, int *wchunk);
 int launch_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m, double lam_d,

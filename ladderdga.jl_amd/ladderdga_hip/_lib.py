@@ -31,7 +31,7 @@ EXPORTED = [
     "ldga_set_kgrid", "ldga_default_wedge",
     "ldga_g_tables", "ldga_g_tables_dev", "ldga_bubble", "ldga_bubble_dev",
     "ldga_chi_gamma", "ldga_chi_gamma_dev",
-    "ldga_lambda0", "ldga_lambda0_dev", "ldga_F_from_chi", "ldga_F_from_chi_dev",
+    "ldga_lambda0", "ldga_lambda0_dev", "ldga_F_from_chi", "ldga_F_from_chi_dev", "ldga_gen_chi", "ldga_gen_chi_dev",
     "ldga_sum_kw", "ldga_sum_kw_dev",
     "ldga_sigma", "ldga_sigma_dev",
     "ldga_malloc", "ldga_free", "ldga_memcpy_h2d", "ldga_memcpy_d2h",
@@ -96,6 +96,8 @@ def lib():
                                          C.POINTER(BseReport), vp]),
             "ldga_lambda0": (i32, [vp, C.POINTER(Freq), dbl, vp, vp, vp]),
             "ldga_F_from_chi": (i32, [vp, C.POINTER(Freq), dbl, i32, vp, vp, vp]),
+            "ldga_gen_chi": (i32, [vp, C.POINTER(Freq), vp, vp, vp]),
+            "ldga_gen_chi_dev": (i32, [vp, C.POINTER(Freq), vp, vp, vp, vp]),
             "ldga_F_from_chi_dev": (i32, [vp, C.POINTER(Freq), dbl, i32, vp, vp, vp, vp]),
             "ldga_lambda0_dev": (i32, [vp, C.POINTER(Freq), dbl, vp, vp, vp, vp]),
             "ldga_sum_kw": (i32, [vp, vp, i32, i32, i32, dbl, dbl, dbl, vp]),

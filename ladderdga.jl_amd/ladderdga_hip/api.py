@@ -103,6 +103,27 @@ def calc_χγ(type_: str, Γr, χ0: χ0T, kG: KGrid, mP: ModelParameters, sP: Si
     return χT(χ, mP.β, tail_c=[0.0, 0.0, mP.Ekin_DMFT]), γT(γ)
 
 
+def calc_gen_χ(Γr, χ0: χ0T, kG: KGrid, ctx=None):
+    """χ[ν,ν',q,ω] = (Γr[:,:,ω] + diag(1/χ₀[q, shell+1:end-shell, ω]))⁻¹ (src/BSETools.jl:259-275), in
+    FP64 on the GPU (global-memory Gauss-Jordan; an analysis function, not the hot path).  Exactly
+    singular matrices give NaN/Inf with a warning, as inv's output would."""
+    ctx = _ctx(ctx, kG)
+    Nν, _, Nω = Γr.shape
+    if Γr.shape != (Nν, Nν, Nω) or Nν % 2 or Nω % 2 == 0:
+        raise ValueError("Γr must be Nν x Nν x Nω with Nν even and Nω odd")
+    sh = χ0.ν_shell_size
+    if χ0.data.shape != (kG.Nq, Nν + 2 * sh, Nω):
+        raise ValueError(f"χ₀ must be {kG.Nq} x {Nν + 2 * sh} x {Nω}")
+    sP = SimulationParameters((Nω - 1) // 2, Nν // 2, sh, 0)
+    out = np.empty((Nν, Nν, kG.Nq, Nω), dtype=c128, order="F")
+    rc = _lib.lib().ldga_gen_chi(ctx.handle, C.byref(_freq(sP)), ptr(Γr, c128, "Γr"), ptr(χ0.data, c128, "χ0.data"),
+                                 out.ctypes.data)
+    ctx.check(rc, allow=(_lib.LDGA_ERR_ZERO_PIVOT,))
+    if rc == _lib.LDGA_ERR_ZERO_PIVOT:
+        warnings.warn("some BSE matrices are exactly singular; their inverses hold NaN/Inf")
+    return out
+
+
 # ---------------------------------------------------------------------------
 # local full vertex (src/BSETools.jl:40-53)
 # ---------------------------------------------------------------------------

@@ -471,3 +471,19 @@ def test_F_from_χ(ctx, kw, diag):
     F = L.F_from_χ(χ, G, sP, 4.0, diag_term=diag, ctx=ctx)
     ref = O.F_from_χ(χ, (G, osP.fft_range.start), osP, 4.0, diag_term=diag)
     assert relerr(F, ref) <= 1e-13
+
+
+@pytest.mark.parametrize("name,kw", CASES[:3] + [("tiny", dict(D=2, Ns=4, n_iν=70, n_iω=1))])
+def test_calc_gen_χ(ctx, name, kw):
+    """calc_gen_χ (BSETools.jl:259-275): the full inverses against numpy's inv, and tied to calc_χγ:
+    Re Σ_νν' χ[ν,ν',q,ω] / β² is the physical χ (BSETools.jl:226-230)."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP = setup(name, **kw)
+    print(f"gen_chi {name} {kw}", flush=True)
+    ref_χ0 = O.calc_bubble(inp.G_fft, inp.G_rfft, oK, omP, osP)
+    χ0 = L.χ0T(np.asfortranarray(ref_χ0), inp.mP.β)
+    X = L.calc_gen_χ(inp.Γ_m, χ0, inp.kG, ctx=ctx)
+    ref = O.calc_gen_χ(inp.Γ_m, ref_χ0)
+    assert relerr(X, ref) <= 1e-10
+    χ, _ = L.calc_χγ("m", inp.Γ_m, χ0, inp.kG, inp.mP, inp.sP, ctx=ctx)
+    assert relerr(np.real(X.sum(axis=(0, 1))) / inp.mP.β ** 2, χ.data) <= 1e-10
