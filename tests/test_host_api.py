@@ -66,3 +66,30 @@ def test_Σ_loc_correction():
     assert np.allclose(out, Σ - 0.25j, rtol=0, atol=1e-15)
     with pytest.raises(ValueError):
         L.Σ_loc_correction(Σ, Σloc[None, :3], Σloc)
+
+
+def test_find_usable_χ_interval_reference_vectors():
+    """The reference's own cases (test/GFFit.jl:43-59), 1-based there, 0-based ranges here."""
+    from ladderdga_hip.types import find_usable_χ_interval as f
+    one = lambda r: list(range(r.start + 1, r.stop + 1))
+    t0 = [0.6, 0.7, 0.8, 0.9, 1.0, 0.9, 0.8, 0.7, 0.6]
+    t1 = [0.8, 0.7, 0.8, 0.9, 1.0, 0.9, 0.8, 0.7, 0.8]
+    t2 = [-0.8, 0.7, 0.8, 0.9, 1.0, 0.9, 0.8, 0.7, -0.8]
+    t3 = [-0.8, 0.7, 0.8, 0.9, 1.0, 0.9, 0.8, 0.7, -0.8, -0.9]
+    t4 = [-0.8, 0.3, -0.2]
+    assert one(f(t1)) == list(range(2, 9))
+    assert one(f(t1, reduce_range_prct=0.0)) == list(range(2, 9))
+    assert one(f(t1, reduce_range_prct=0.50)) == list(range(3, 8))
+    assert one(f(t2, reduce_range_prct=0.0)) == list(range(2, 9))
+    with pytest.raises(ValueError):
+        f(t3)
+    assert one(f(t0, reduce_range_prct=0.0)) == list(range(1, 10))
+    assert one(f(t2, sum_type="full", reduce_range_prct=0.5)) == list(range(1, 10))
+    assert one(f(t2, sum_type=(2, 8), reduce_range_prct=0.5)) == list(range(2, 9))
+    assert one(f([-1.0, -2.0, -1.0])) == [2]
+    assert one(f(t4, reduce_range_prct=0.9)) == [2]
+    # over q: the intersection of the rows, each at reduce_range_prct 0.1 (GFFit.jl:71-73)
+    import ladderdga_hip as L
+    χ = L.χT(np.array([t0, t1]), 5.0, full_range=False)
+    assert one(χ.usable_ω) == list(range(2, 9))
+    assert list(L.χT(np.array([t0, t1]), 5.0).usable_ω) == list(range(9))
