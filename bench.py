@@ -142,6 +142,31 @@ def lambdadm_step_costs(L, ctx, o, λ0, inp, numax, Nq, Nν, Nω, reps=5):
             "step_speedup": 3 * t_res / t_jac, "mu_iters": int(r["mu_iters"])}
 
 
+def run_sc_iteration_cost(L, ctx, o, λ0, inp, numax, Nq, Nν, Nω):
+    """Cost of one iteration of the self-consistency loop (run_sc!, conditions_new.jl:479-491) on this
+    step's ladder arrays: EoM, mixing, μ fit and energies, G-table rebuild, all resident on the device.
+    Taken as (t(5 iterations) - t(1 iteration)) / 4 of run_sc type :fix, so the one-time uploads
+    cancel; conv_abs = 0 keeps the loop from stopping early."""
+    import numpy as np
+    Fo = lambda a, shape: np.asfortranarray(a.cpu().numpy().reshape(shape, order="F"))
+    χm = L.χT(Fo(o["χm"], (Nq, Nω)), inp.mP.β, tail_c=[0, 0, inp.mP.Ekin_DMFT])
+    χd = L.χT(Fo(o["χd"], (Nq, Nω)), inp.mP.β, tail_c=[0, 0, inp.mP.Ekin_DMFT])
+    γm, γd = L.γT(Fo(o["γm"], (Nq, Nν, Nω))), L.γT(Fo(o["γd"], (Nq, Nν, Nω)))
+    l0 = Fo(λ0, (Nq, Nν, Nω))
+    lam = (2 * abs(L.get_λ_min(χm.data)) + 1.0, 2 * abs(L.get_λ_min(χd.data)) + 1.0)
+
+    def t(maxit):
+        t0 = time.perf_counter()
+        r = L.run_sc(χm, γm, χd, γd, l0, inp.mP.μ, inp.Σ_loc, inp.G_rfft, inp.χloc_m_sum, inp.kG, inp.mP, inp.sP,
+                     type="fix", νmax=numax, maxit=maxit, mixing=0.2, conv_abs=0.0, λm=lam[0], λd=lam[1],
+                     trace=True, ctx=ctx)
+        return time.perf_counter() - t0, len(r.trace)
+
+    t(1)
+    (t1, n1), (t5, n5) = t(1), t(5)
+    return {"iteration_ms": (t5 - t1) / max(n5 - n1, 1) * 1e3, "iterations_timed": n5 - n1, "lambda": list(lam)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -275,6 +300,7 @@ def main():
         "mixed_rel_err_vs_fp64": errs,
     }
     result["lambdadm_newton_step"] = lambdadm_step_costs(L, ctx, out[main_prec], λ0, inp, numax, Nq, Nν, Nω)
+    result["run_sc_iteration"] = run_sc_iteration_cost(L, ctx, out[main_prec], λ0, inp, numax, Nq, Nν, Nω)
     # G tables (helpers.jl:153-183) from the wedge G_loc, as the self-consistency loop rebuilds them:
     # algorithmic bytes = wedge G read + both full tables written
     gw = up(inp.gLoc)
