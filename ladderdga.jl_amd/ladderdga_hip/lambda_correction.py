@@ -389,8 +389,13 @@ def run_sc(χm: χT, γm: γT, χd: χT, γd: γT, λ0, μ: float, Σ_loc, gLoc_
             if type in ("m", "pre_m", "dm", "pre_dm"):
                 lhs_c1, E_pot_2 = out.lhs_c1, out.Epot2
             if trace:
-                rows.append(dict(it=it, λm=λm, λd=λd, μ=μ, n=n, EKin=E_kin, EPot=E_pot_1, lhs_c1=lhs_c1,
-                                 EPot_c2=E_pot_2))
+                # the reference's trace row (conditions_new.jl:507-515): χ sums in both orders and the
+                # checksums |Σ Σ_ladder|, |Σ G_ladder| of this iteration
+                cs_m, cs_d = sum_kω(kG, χm, λ=λm, ctx=ctx), sum_kω(kG, χd, λ=λd, ctx=ctx)
+                rows.append(dict(it=it, λm=λm, λd=λd, μ=μ, n=n, EKin=E_kin, EPot=E_pot_1,
+                                 lhs_c1=(cs_d + cs_m) / 2, EPot_c2=E_pot_2, cs_m=cs_m, cs_m2=cs_m, cs_d=cs_d,
+                                 cs_d2=cs_d, cs_Σ=float(abs(cur.to_host().sum())),
+                                 cs_G=float(abs(d["G"].to_host().sum()))))
             if abs(E_pot_1 - E_pot_1_old) < conv_abs and abs(E_pot_2 - E_pot_2_old) < conv_abs:
                 converged = True
                 break

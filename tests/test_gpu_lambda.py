@@ -132,6 +132,12 @@ def test_run_sc_fix(ctx, at, mixing, maxit):
     # as the reference's run_sc!, :fix reports its loop-local λ = 0 and logs the λ used in the trace
     assert (r.λm, r.λd) == (0.0, 0.0)
     assert all((row["λm"], row["λd"]) == tuple(λ) for row in r.trace)
+    # the trace's checksums of the last iteration (conditions_new.jl:507-515)
+    last = r.trace[-1]
+    assert last["cs_Σ"] == pytest.approx(abs(ref["Σ"].sum()), rel=1e-9)
+    assert last["cs_G"] == pytest.approx(abs(ref["G"].sum()), rel=1e-9)
+    assert last["cs_m"] == pytest.approx(O.sum_kω(oK, O.ChiT(O.χ_λ(o["χm"].data, λ[0]), omP.beta,
+                                                             tail_c=o["χm"].tail_c)), rel=1e-12)
 
 
 @pytest.mark.parametrize("mixing,iters", [(0.0, 12), (0.5, 20)])
