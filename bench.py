@@ -159,8 +159,9 @@ def run_sc_iteration_cost(L, ctx, o, λ0, inp, numax, Nq, Nν, Nω):
         t0 = time.perf_counter()
         r = L.run_sc(χm, γm, χd, γd, l0, inp.mP.μ, inp.Σ_loc, inp.G_rfft, inp.χloc_m_sum, inp.kG, inp.mP, inp.sP,
                      type="fix", νmax=numax, maxit=maxit, mixing=0.2, conv_abs=0.0, λm=lam[0], λd=lam[1],
-                     trace=True, ctx=ctx)
-        return time.perf_counter() - t0, len(r.trace)
+                     ctx=ctx)
+        # conv_abs = 0 never stops early; a failed μ fit would (NaN μ) and is reported
+        return time.perf_counter() - t0, maxit if np.isfinite(r.μ) else 0
 
     t(1)
     (t1, n1), (t5, n5) = t(1), t(5)
