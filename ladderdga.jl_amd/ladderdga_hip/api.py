@@ -240,6 +240,35 @@ def Σ_loc_correction(Σ_ladder, Σ_ladderLoc, Σ_loc):
     return Σ_ladder - Σ_ladderLoc[0, :νmax][None, :] + np.asarray(Σ_loc)[:νmax][None, :]
 
 
+def update_tail(χ: χT, new_tail_c, ωnGrid=None) -> None:
+    """update_tail!(χ, new_tail_c, ωnGrid) (src/DataTypes.jl:315-345): replaces the c₂/ω² tail of χ,
+    χ[:, ω] += (c₂_new - c₂_old)/(iω_n)² for ω ≠ 0, with χ's λ taken off and put back around it.  On
+    the host, as in the reference: O(Nq Nω) additions on the caller's array."""
+    new_tail_c = [float(c) for c in new_tail_c]
+    Nω = χ.data.shape[1]
+    if ωnGrid is None:
+        ωnGrid = 2j * np.pi * np.asarray(χ.indices_ω) / χ.β
+    ωnGrid = np.asarray(ωnGrid)
+    if len(new_tail_c) != len(χ.tail_c):
+        raise ValueError(f"length of new tail coefficient array ({len(new_tail_c)}) must be the same as the old "
+                         f"one ({len(χ.tail_c)})!")
+    if len(ωnGrid) != Nω:
+        raise ValueError(f"length of ωnGrid ({len(ωnGrid)}) must be the same as for χ ({Nω})!")
+    if not all(math.isfinite(c) for c in new_tail_c):
+        raise ValueError("One or more tail coefficients are not finite!")
+    if sum(new_tail_c) != new_tail_c[2]:
+        raise ValueError("update tail only implemented for c_2 coefficient!")
+    λ_bak = χ.λ
+    if λ_bak != 0:
+        reset(χ)
+    if new_tail_c[2] != χ.tail_c[2]:
+        ω = np.abs(ωnGrid) >= 1e-10
+        χ.data[:, ω] += np.real((new_tail_c[2] - χ.tail_c[2]) / ωnGrid[ω] ** 2)[None, :]
+        χ.tail_c[2] = new_tail_c[2]
+    if λ_bak != 0:
+        χ_λ_inplace(χ, λ_bak)
+
+
 def get_λ_min(χr: np.ndarray) -> float:
     """-min_q 1/χ(q, ω₀) (LambdaCorrection/helpers.jl:217-220)."""
     nh = math.ceil(χr.shape[1] / 2) - 1

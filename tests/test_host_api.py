@@ -93,3 +93,32 @@ def test_find_usable_χ_interval_reference_vectors():
     χ = L.χT(np.array([t0, t1]), 5.0, full_range=False)
     assert one(χ.usable_ω) == list(range(2, 9))
     assert list(L.χT(np.array([t0, t1]), 5.0).usable_ω) == list(range(9))
+
+
+def test_update_tail():
+    """update_tail! (DataTypes.jl:315-345): χ[:, ω] gains (c₂' - c₂)/(iω_n)² off ω = 0, the tail
+    coefficient follows, a λ on χ is kept; and the tail-corrected sum changes by exactly -βΔ/12 (the
+    analytic Σ_n≠0 1/(iω_n)² = -β²/12 of the tail term), checked with the oracle's sum."""
+    import ladderdga_hip as L
+    from oracle import ldga_oracle as O
+    rng = np.random.default_rng(4)
+    β, c, Δ = 7.0, 0.4, 0.15
+    d = rng.uniform(0.1, 0.5, (6, 9))
+    χ = L.χT(np.asfortranarray(d.copy()), β, tail_c=[0.0, 0.0, c])
+    L.update_tail(χ, [0.0, 0.0, c + Δ])
+    ωn = 2j * np.pi * np.arange(-4, 5) / β
+    exp = d.copy()
+    exp[:, np.arange(9) != 4] += np.real(Δ / ωn[np.arange(9) != 4] ** 2)
+    assert np.allclose(χ.data, exp, rtol=0, atol=1e-15) and χ.tail_c[2] == c + Δ
+    kG = O.KGrid(2, 4, 1.0)
+    assert kG.Nq == 6
+    s0 = O.sum_kω(kG, O.ChiT(d, β, tail_c=[0, 0, c]))
+    s1 = O.sum_kω(kG, O.ChiT(np.asarray(χ.data), β, tail_c=[0, 0, c + Δ]))
+    assert s1 - s0 == pytest.approx(-β * Δ / 12, rel=1e-12)
+    # with a λ on χ: taken off, the tail replaced, put back
+    χ2 = L.χT(np.asfortranarray(d.copy()), β, tail_c=[0.0, 0.0, c])
+    L.χ_λ_inplace(χ2, 0.3)
+    L.update_tail(χ2, [0.0, 0.0, c + Δ])
+    assert χ2.λ == 0.3 and np.allclose(χ2.data, L.χ_λ(exp, 0.3), rtol=1e-14, atol=1e-15)
+    with pytest.raises(ValueError, match="c_2"):
+        L.update_tail(χ2, [0.0, 0.1, c])
