@@ -59,9 +59,15 @@ typedef struct {
   int32_t n_fft;       /* length(sP.fft_range)                           */
 } ldga_freq;
 
-/* ModelParameters (src/Config.jl:39-47) fields used by the path. */
+/* ModelParameters (src/Config.jl:39-47) fields used by the path, and the
+ * 1/ω² tail coefficients of the two susceptibilities the path sums
+ * (chi_m.tail_c[3] and chi_d.tail_c[3], src/DataTypes.jl:177-194): the c2 of
+ * sum_kω(χm) in the EoM tail and of sum_kω(χr) in the λdm residual
+ * (DataTypes.jl:215-249).  calc_χγ sets both to Ekin_DMFT
+ * (BSETools.jl:241-243); update_tail! (DataTypes.jl:315-345) changes them. */
 typedef struct {
   double U, mu, beta, n, Ekin_DMFT;
+  double chi_m_c2, chi_d_c2;
 } ldga_model;
 
 /* Per-call BSE report. */
@@ -177,12 +183,28 @@ int ldga_sum_kw(ldga_ctx *ctx, const double *chi, int Nw, int w_lo, int w_hi, do
 int ldga_sum_kw_dev(ldga_ctx *ctx, const void *chi, int Nw, int w_lo, int w_hi, double c2, double beta,
                     double lam, void *out_dev, void *stream);
 
+/* ---- sum_ω / sum_ωk (src/DataTypes.jl:257-308) ---------------------------
+ * Per q, over the usable slice w_lo..w_hi (0-based, inclusive) of chi
+ * (Nq x Nw), as the reference's sum_ω! / sum_ω:
+ *   res[q] = (sum_{n=0}^{L-1} [chi_λ(q, w_lo + n) - t(n)] - c2 beta^2/12) / beta,
+ * L = w_hi - w_lo + 1, t(n) = Re(c2 / (i w_m)^2) with m = n - n_iw and t = 0
+ * at m = 0: slice element n is paired with the n-th frequency of the whole
+ * grid, exactly as DataTypes.jl:293-305 indexes ωn_arr (on the full range
+ * this is each element's own frequency).  chi_λ = chi/(lam chi + 1), lam = 0
+ * meaning no map (sum_ωk's χ_λ!, DataTypes.jl:257-264).  out_q (Nq doubles,
+ * nullable) receives res; out_k (1 double, nullable; needs the k grid, whose
+ * Nq must match) receives kintegrate(res) = sum_ωk. */
+int ldga_sum_w(ldga_ctx *ctx, const double *chi, int Nq, int Nw, int w_lo, int w_hi, double c2, double beta,
+               double lam, double *out_q, double *out_k);
+int ldga_sum_w_dev(ldga_ctx *ctx, const void *chi, int Nq, int Nw, int w_lo, int w_hi, double c2, double beta,
+                   double lam, void *out_q, void *out_k, void *stream);
+
 /* ---- EoM self-energy: calc_Σ (src/ladderDGATools_singleCore.jl:114-136) -
  * Sigma[k, nu] (Nq x numax, nu = 0..numax-1) =
  *   (1/beta) sum_w conv_q(K(q,nu,w), G(k+q, nu+w)) + tail/(i nu_n) + n U/2,
  * K = eom(U, gamma_m, gamma_d, chi_m^λm, chi_d^λd, lambda0)
  * (src/ladderDGATools.jl:260), tail = -U (sum_kω(chi_m^λm) - chiloc_m_sum)
- * when tc != 0.  eom_kind selects one of the split kernels (LDGA_EOM_*) used
+ * (tail coefficient mp->chi_m_c2) when tc != 0.  eom_kind selects one of the split kernels (LDGA_EOM_*) used
  * by calc_Σ_parts; for those the Hartree term is added only to REST and the
  * tail only when tc != 0 and eom_kind == FULL.  chi_m/chi_d (Nq x Nw) are
  * read-only: the lambda map is applied on the fly and never stored. */
@@ -204,9 +226,12 @@ double ldga_G_shell_sum(int N, double beta);
 /* ---- lambda_dm residual: residual_vals / residual_f of λdm_correction ------
  * (LambdaCorrection/conditions_new.jl:250-287).  For (lam_m, lam_d):
  * Sigma = calc_Σ(...; λm, λd, tc) over nu = 0..numax-1, mu refit so that
- * filling_pos(G_ladder[:, 0:numax-1]) = n (GFTools.jl:162-186; secant to
- * |f| < 1e-14 when fit_mu != 0, else mu = mu0), E_kin/E_pot1 from calc_E
- * (thermodynamics.jl:142-168), the chi sums sum_kω(chi_r^λr) and
+ * filling_pos(G_ladder[:, 0:numax-1]) = n (GFTools.jl:162-186; solved to
+ * |f| < 1e-14 when fit_mu != 0, else mu = mu0; a fit that ends with
+ * |f| > 1e-8, the reference's find_zero tolerance, gives mu = NaN and NaN
+ * results, as G_from_Σladder! returns NaN), E_kin/E_pot1 from calc_E
+ * (thermodynamics.jl:142-168), the chi sums sum_kω(chi_r^λr) (tail
+ * coefficients mp->chi_m_c2, mp->chi_d_c2) and
  *   res_pp   = (Σχd + Σχm)/2 - n/2 (1 - n/2)
  *   res_epot = E_pot1 - [(U/2)(Σχd - Σχm) + U n^2/4].
  * want_jac != 0 also returns d(res_pp, res_epot)/d(lam_m, lam_d) from the
@@ -219,7 +244,9 @@ typedef struct {
   double n, mu, Ekin, Epot1, Epot2, lhs_c1, res_pp, res_epot;
   double jac[4]; /* row-major: [dres_pp/dlam_m, dres_pp/dlam_d, dres_epot/dlam_m, dres_epot/dlam_d] */
   int32_t mu_iters;
-  int32_t mu_converged;
+  int32_t mu_converged; /* 1 when |f(mu)| <= 1e-8 (or no fit was asked for) */
+  double n_full;        /* filling(G_ladder, kG, U, mu, beta) over the whole fft_range (GFTools.jl:234-241),
+                           the trace's n (conditions_new.jl:511); NaN when Sigma_loc is not given */
 } ldga_lambdadm_out;
 
 int ldga_lambdadm_residual(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m,

@@ -272,21 +272,25 @@ int launch_bse_mixed(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, co
                      int *redo, hipStream_t st);
 int launch_bse_blk_f64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam,
                        const double2 *chi0, double s, double U, double beta, double *chi, double2 *gamma,
-                       unsigned long long *counters, const int *only, int n_only, hipStream_t st);
+                       unsigned long long *cnt, const int *list, const unsigned long long *count, int max_items,
+                       hipStream_t st);
 int launch_bse_gmem_f64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam,
                         const double2 *chi0, double s, double U, double beta, double *chi, double2 *gamma,
-                        unsigned long long *cnt, const int *only, int n_only, hipStream_t st,
-                        double2 *inv);
+                        unsigned long long *cnt, const int *only, int n_only, const unsigned long long *n_dev,
+                        hipStream_t st, double2 *inv);
 
-// FP64 over every matrix or the listed ones: the blocked MFMA kernel up to
-// Nnu = 128, the global-memory kernel above.
+// FP64 over every matrix (list == NULL) or over the list[0 .. *count - 1] of
+// at most max_items matrices, count read on the device: the blocked MFMA
+// kernel up to Nnu = 128, the global-memory kernel above.
 static int launch_fp64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam,
                        const double2 *chi0, double s, double U, double beta, double *chi, double2 *gamma,
-                       unsigned long long *cnt, const int *only, int n_only, hipStream_t st) {
+                       unsigned long long *cnt, const int *list, const unsigned long long *count, int max_items,
+                       hipStream_t st) {
   if (N <= 128)
-    return launch_bse_blk_f64(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, chi, gamma, cnt, only, n_only, st);
-  return launch_bse_gmem_f64(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, chi, gamma, cnt, only, n_only, st,
-                             nullptr);
+    return launch_bse_blk_f64(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, chi, gamma, cnt, list, count,
+                              max_items, st);
+  return launch_bse_gmem_f64(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, chi, gamma, cnt, list,
+                             list ? max_items : 0, count, st, nullptr);
 }
 
 // Clears the report counters with a kernel rather than hipMemsetAsync, so that
@@ -310,7 +314,7 @@ int launch_gen_chi(ldga_ctx *ctx, const ldga_freq *fr, const double2 *Gam, const
   bse_clear_counters<<<1, 8, 0, st>>>(cnt, 8);
   LDGA_HIP_TRY(ctx, hipGetLastError());
   int rc = launch_bse_gmem_f64(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, 1.0, 0.0, 1.0, nullptr, nullptr, cnt,
-                               nullptr, 0, st, inv);
+                               nullptr, 0, nullptr, st, inv);
   if (rc) return rc;
   unsigned long long zeros = 0;
   LDGA_HIP_TRY(ctx, hipMemcpyAsync(&zeros, cnt, 8, hipMemcpyDeviceToHost, st));
@@ -345,7 +349,8 @@ int launch_bse(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double 
   if (resolve && !(redo = (int *)ldga_buf(ctx, "bse_redo", (size_t)Nq * Nw * sizeof(int)))) return LDGA_ERR_HIP;
   int rc;
   if (precision == LDGA_PREC_FP64 && !use_register_gj()) {
-    rc = launch_fp64(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, nullptr, 0, st);
+    rc = launch_fp64(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, nullptr, nullptr, 0,
+                     st);
   } else if (precision == LDGA_PREC_FP64) {
     if (N <= 47)
       rc = run_gj<3, 3, 16>(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, st);
@@ -359,35 +364,31 @@ int launch_bse(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double 
       return ldga_fail(ctx, LDGA_ERR_SHAPE, "the register Gauss-Jordan kernel (LDGA_BSE_F64=gj) supports Nnu <= 127 (got %d)", N);
   } else if (precision == LDGA_PREC_MIXED) {
     rc = launch_bse_mixed(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, redo, st);
+    // the refinement failures listed by the mixed kernel (their number in
+    // cnt[2], read on the device) are re-solved in FP64, overwriting their
+    // chi and gamma; no host synchronisation
+    if (!rc)
+      rc = launch_fp64(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt + 8, redo, cnt + 2,
+                       Nq * Nw, st);
   } else {
     return ldga_fail(ctx, LDGA_ERR_ARG, "unknown precision mode %d", precision);
   }
   if (rc) return rc;
+  if (!rep) return LDGA_OK;  // asynchronous: zero pivots show only as NaN outputs
   unsigned long long h[6] = {0, 0, 0, 0, 0, 0}, h2[6] = {0, 0, 0, 0, 0, 0};
-  if (rep || resolve) {
-    LDGA_HIP_TRY(ctx, hipMemcpyAsync(h, cnt, sizeof h, hipMemcpyDeviceToHost, st));
-    LDGA_HIP_TRY(ctx, hipStreamSynchronize(st));
-  }
-  if (resolve && h[2]) {
-    // the listed matrices' chi and gamma are overwritten by the FP64 kernel
-    rc = launch_fp64(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt + 8, redo, (int)h[2],
-                     st);
-    if (rc) return rc;
-    LDGA_HIP_TRY(ctx, hipMemcpyAsync(h2, cnt + 8, sizeof h2, hipMemcpyDeviceToHost, st));
-    LDGA_HIP_TRY(ctx, hipStreamSynchronize(st));
-  }
+  LDGA_HIP_TRY(ctx, hipMemcpyAsync(h, cnt, sizeof h, hipMemcpyDeviceToHost, st));
+  if (resolve) LDGA_HIP_TRY(ctx, hipMemcpyAsync(h2, cnt + 8, sizeof h2, hipMemcpyDeviceToHost, st));
+  LDGA_HIP_TRY(ctx, hipStreamSynchronize(st));
   const unsigned long long nzero = h[0] + h2[0];
-  if (rep) {
-    rep->n_matrices = (int64_t)Nq * Nw;
-    rep->n_zero_pivot = (int64_t)nzero;
-    rep->ir_steps_max = (int32_t)h[1];
-    rep->n_fallback = (int32_t)h[2];
-    double r;
-    memcpy(&r, &h[3], 8);
-    rep->max_rel_resid = r;
-    rep->ir_steps_mean = (double)h[5] / (double)((int64_t)Nq * Nw);
-    rep->n_fp64_resolved = resolve ? (int64_t)h[2] : 0;
-  }
+  rep->n_matrices = (int64_t)Nq * Nw;
+  rep->n_zero_pivot = (int64_t)nzero;
+  rep->ir_steps_max = (int32_t)h[1];
+  rep->n_fallback = (int32_t)h[2];
+  double r;
+  memcpy(&r, &h[3], 8);
+  rep->max_rel_resid = r;
+  rep->ir_steps_mean = (double)h[5] / (double)((int64_t)Nq * Nw);
+  rep->n_fp64_resolved = resolve ? (int64_t)h[2] : 0;
   if (nzero) return ldga_fail(ctx, LDGA_ERR_ZERO_PIVOT, "%llu BSE matrices had an exactly zero pivot", nzero);
   return LDGA_OK;
 }

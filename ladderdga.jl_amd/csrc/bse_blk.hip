@@ -36,6 +36,7 @@
 // extrapolated from the contraction rate, is <= 2^-48 |y|_inf, a few units in
 // the last place) or the correction stops halving (the FP64 residual's
 // rounding level, kappa(A) 2^-53, is reached).
+#include <algorithm>
 #include <climits>
 #include <cstring>
 #include <type_traits>
@@ -76,24 +77,19 @@ struct BlkShared {
 template <typename T, int NT>
 constexpr int blk_min_waves() { return (sizeof(T) == 4 && NT <= 8) ? 4 : 1; }
 
+// One workgroup solves matrix lb = q + Nq w (the body of both kernels below).
 template <typename T, int NT, bool REFINE>
-__global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_kernel(int N, int Nq, int Nc, int shell,
-                                                          const double2 *__restrict__ Gam,
-                                                          const double2 *__restrict__ chi0, double s, double U,
-                                                          double beta, int max_ir, double *__restrict__ chi_out,
-                                                          double2 *__restrict__ gamma_out,
-                                                          unsigned long long *__restrict__ counters,
-                                                          int *__restrict__ redo, const int *__restrict__ only) {
+__device__ __forceinline__ void blk_solve(BlkShared<T, NT> &sh, const int lb, int N, int Nq, int Nc, int shell,
+                                          const double2 *__restrict__ Gam, const double2 *__restrict__ chi0, double s,
+                                          double U, double beta, int max_ir, double *__restrict__ chi_out,
+                                          double2 *__restrict__ gamma_out, unsigned long long *__restrict__ counters,
+                                          int *__restrict__ redo) {
   typedef mfma_traits<T> MT;
   typedef typename MT::v4 v4;
   typedef typename MT::c2 c2;
   constexpr int NC = 16 * NT, NU = (NC + 63) / 64;
-  __shared__ BlkShared<T, NT> sh;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, n = lane & 15;
-  // only != NULL: block b solves matrix only[b] (the FP64 re-solve of a
-  // refinement's failures); otherwise every (q, w), XCD-aware
-  const int lb = only ? only[blockIdx.x] : xcd_logical_block(blockIdx.x, gridDim.x);
   const int q = lb % Nq, w = lb / Nq;
   const double2 *G = Gam + (size_t)w * N * N;
 
@@ -632,44 +628,88 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_ker
   }
 }
 
+// every (q, w), one workgroup each, XCD-aware
+template <typename T, int NT, bool REFINE>
+__global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_kernel(int N, int Nq, int Nc, int shell,
+                                                          const double2 *__restrict__ Gam,
+                                                          const double2 *__restrict__ chi0, double s, double U,
+                                                          double beta, int max_ir, double *__restrict__ chi_out,
+                                                          double2 *__restrict__ gamma_out,
+                                                          unsigned long long *__restrict__ counters,
+                                                          int *__restrict__ redo) {
+  __shared__ BlkShared<T, NT> sh;
+  blk_solve<T, NT, REFINE>(sh, xcd_logical_block(blockIdx.x, gridDim.x), N, Nq, Nc, shell, Gam, chi0, s, U, beta,
+                           max_ir, chi_out, gamma_out, counters, redo);
+}
+
+// FP64 over the matrices listed in list[0 .. *count - 1] (q + Nq w each): the
+// re-solve of a mixed call's refinement failures.  The count is read on the
+// device, so the host never waits for it; each workgroup takes the items
+// blockIdx.x, blockIdx.x + gridDim.x, ...
+template <typename T, int NT>
+__global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_list_kernel(
+    int N, int Nq, int Nc, int shell, const double2 *__restrict__ Gam, const double2 *__restrict__ chi0, double s,
+    double U, double beta, double *__restrict__ chi_out, double2 *__restrict__ gamma_out,
+    unsigned long long *__restrict__ counters, const int *__restrict__ list,
+    const unsigned long long *__restrict__ count) {
+  __shared__ BlkShared<T, NT> sh;
+  const int nitems = __builtin_amdgcn_readfirstlane((int)*count);
+  for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+    blk_solve<T, NT, false>(sh, list[item], N, Nq, Nc, shell, Gam, chi0, s, U, beta, 0, chi_out, gamma_out, counters,
+                            nullptr);
+    __syncthreads();  // the shared state is reused by the next item
+  }
+}
+
 template <typename T, int NT, bool REFINE>
 int run_blk(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam, const double2 *chi0,
             double s, double U, double beta, int max_ir, double *chi, double2 *gamma, unsigned long long *cnt,
-            int *redo, const int *only, int n_only, hipStream_t st) {
+            int *redo, const int *list, const unsigned long long *count, int max_items, hipStream_t st) {
   if (N > 16 * NT) return ldga_fail(ctx, LDGA_ERR_SHAPE, "BSE tile too small for N=%d", N);
-  const unsigned nblocks = only ? (unsigned)n_only : (unsigned)Nq * Nw;
+  if (list) {
+    if constexpr (!REFINE) {
+      if (max_items <= 0) return LDGA_OK;
+      const unsigned grid = (unsigned)std::min(max_items, 1024);
+      bse_blk_list_kernel<T, NT><<<dim3(grid), dim3(64 * NT), 0, st>>>(N, Nq, Nc, shell, Gam, chi0, s, U, beta, chi,
+                                                                        gamma, cnt, list, count);
+      LDGA_HIP_TRY(ctx, hipGetLastError());
+      return LDGA_OK;
+    }
+    return ldga_fail(ctx, LDGA_ERR_ARG, "a matrix list is solved in FP64 only");
+  }
+  const unsigned nblocks = (unsigned)Nq * Nw;
   if (nblocks == 0) return LDGA_OK;
   bse_blk_kernel<T, NT, REFINE><<<dim3(nblocks), dim3(64 * NT), 0, st>>>(N, Nq, Nc, shell, Gam, chi0, s, U, beta,
-                                                                         max_ir, chi, gamma, cnt, redo, only);
+                                                                         max_ir, chi, gamma, cnt, redo);
   LDGA_HIP_TRY(ctx, hipGetLastError());
   return LDGA_OK;
 }
 
 }  // namespace
 
-// FP64 blocked kernel: Nnu <= 128.  With only != NULL it solves just the
-// n_only matrices (q + Nq w) listed there.
+// FP64 blocked kernel: Nnu <= 128.  With list != NULL it solves just the
+// matrices (q + Nq w) list[0 .. *count - 1], count read on the device, at most
+// max_items of them.
 int launch_bse_blk_f64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam,
                        const double2 *chi0, double s, double U, double beta, double *chi, double2 *gamma,
-                       unsigned long long *cnt, const int *only, int n_only, hipStream_t st) {
+                       unsigned long long *cnt, const int *list, const unsigned long long *count, int max_items,
+                       hipStream_t st) {
   const int nt = (N + 15) / 16;
+#define LDGA_BLK_F64(NT_)                                                                                     \
+  return run_blk<double, NT_, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, nullptr, \
+                                     list, count, max_items, st)
   switch (nt) {
     case 1:
     case 2:
-    case 3: return run_blk<double, 3, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, nullptr,
-                                                only, n_only, st);
-    case 4: return run_blk<double, 4, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, nullptr,
-                                                only, n_only, st);
-    case 5: return run_blk<double, 5, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, nullptr,
-                                                only, n_only, st);
-    case 6: return run_blk<double, 6, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, nullptr,
-                                                only, n_only, st);
-    case 7: return run_blk<double, 7, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, nullptr,
-                                                only, n_only, st);
-    case 8: return run_blk<double, 8, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, nullptr,
-                                                only, n_only, st);
+    case 3: LDGA_BLK_F64(3);
+    case 4: LDGA_BLK_F64(4);
+    case 5: LDGA_BLK_F64(5);
+    case 6: LDGA_BLK_F64(6);
+    case 7: LDGA_BLK_F64(7);
+    case 8: LDGA_BLK_F64(8);
     default: return ldga_fail(ctx, LDGA_ERR_SHAPE, "FP64 blocked BSE supports Nnu <= 128 (got %d)", N);
   }
+#undef LDGA_BLK_F64
 }
 
 // Mixed: FP32 inverse on MFMA + FP64 refinement, Nnu <= 160.  With redo !=
@@ -679,25 +719,21 @@ int launch_bse_mixed(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, co
                      double s, double U, double beta, double *chi, double2 *gamma, unsigned long long *cnt,
                      int *redo, hipStream_t st) {
   const int nt = (N + 15) / 16, ir = 6;
+#define LDGA_BLK_MIXED(NT_)                                                                                   \
+  return run_blk<float, NT_, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo, \
+                                   nullptr, nullptr, 0, st)
   switch (nt) {
     case 1:
     case 2:
-    case 3: return run_blk<float, 3, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo,
-                                                nullptr, 0, st);
-    case 4: return run_blk<float, 4, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo,
-                                                nullptr, 0, st);
-    case 5: return run_blk<float, 5, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo,
-                                                nullptr, 0, st);
-    case 6: return run_blk<float, 6, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo,
-                                                nullptr, 0, st);
-    case 7: return run_blk<float, 7, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo,
-                                                nullptr, 0, st);
-    case 8: return run_blk<float, 8, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo,
-                                                nullptr, 0, st);
-    case 9: return run_blk<float, 9, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo,
-                                                nullptr, 0, st);
-    case 10: return run_blk<float, 10, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo,
-                                                nullptr, 0, st);
+    case 3: LDGA_BLK_MIXED(3);
+    case 4: LDGA_BLK_MIXED(4);
+    case 5: LDGA_BLK_MIXED(5);
+    case 6: LDGA_BLK_MIXED(6);
+    case 7: LDGA_BLK_MIXED(7);
+    case 8: LDGA_BLK_MIXED(8);
+    case 9: LDGA_BLK_MIXED(9);
+    case 10: LDGA_BLK_MIXED(10);
     default: return ldga_fail(ctx, LDGA_ERR_SHAPE, "mixed BSE supports Nnu <= 160 (got %d)", N);
   }
+#undef LDGA_BLK_MIXED
 }

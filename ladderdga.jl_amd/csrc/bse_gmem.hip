@@ -29,6 +29,7 @@ __global__ __launch_bounds__(GM_THREADS) void bse_gmem_f64_kernel(int N, int Nq,
                                                                   double2 *__restrict__ gamma_out,
                                                                   unsigned long long *__restrict__ counters,
                                                                   const int *__restrict__ only, int n_items,
+                                                                  const unsigned long long *__restrict__ n_dev,
                                                                   double2 *__restrict__ ws,
                                                                   double2 *__restrict__ inv_out) {
   extern __shared__ double2 lds[];
@@ -46,6 +47,8 @@ __global__ __launch_bounds__(GM_THREADS) void bse_gmem_f64_kernel(int N, int Nq,
   double2 *M = ws + (size_t)blockIdx.x * N * N;
   // this thread's first (row, column) of the flat row-major walk, and the step
   const int i0 = tid / N, c0 = tid - i0 * N, di = GM_THREADS / N, dc = GM_THREADS - di * N;
+  // a list whose length is known only on the device: at most n_items of it
+  if (n_dev) n_items = min(n_items, __builtin_amdgcn_readfirstlane((int)*n_dev));
 
   for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
     const int lb = only ? only[item] : item;
@@ -173,12 +176,14 @@ __global__ __launch_bounds__(GM_THREADS) void bse_gmem_f64_kernel(int N, int Nq,
 
 }  // namespace
 
-// FP64 Gauss-Jordan in global memory: every (q, w), or the n_only matrices
-// (q + Nq w) listed in only.  Any Nnu up to 1024.  chi and gamma may be NULL;
-// inv (NULL or Nnu x Nnu x Nq x Nw) receives the full inverses.
+// FP64 Gauss-Jordan in global memory: every (q, w), or the matrices (q + Nq w)
+// listed in only: n_only of them, or, with n_dev != NULL, the count *n_dev read
+// on the device (at most n_only).  Any Nnu up to 1024.  chi and gamma may be
+// NULL; inv (NULL or Nnu x Nnu x Nq x Nw) receives the full inverses.
 int launch_bse_gmem_f64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam,
                         const double2 *chi0, double s, double U, double beta, double *chi, double2 *gamma,
-                        unsigned long long *cnt, const int *only, int n_only, hipStream_t st, double2 *inv) {
+                        unsigned long long *cnt, const int *only, int n_only, const unsigned long long *n_dev,
+                        hipStream_t st, double2 *inv) {
   if (N < 1 || N > 1024) return ldga_fail(ctx, LDGA_ERR_SHAPE, "FP64 global-memory BSE supports Nnu <= 1024 (got %d)", N);
   const int n_items = only ? n_only : Nq * Nw;
   if (n_items == 0) return LDGA_OK;
@@ -190,7 +195,8 @@ int launch_bse_gmem_f64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell,
   if (!ws) return LDGA_ERR_HIP;
   const size_t lds = (size_t)N * (3 * sizeof(double2) + 2 * sizeof(int));
   bse_gmem_f64_kernel<<<dim3((unsigned)grid), dim3(GM_THREADS), lds, st>>>(N, Nq, Nc, shell, Gam, chi0, s, U, beta,
-                                                                          chi, gamma, cnt, only, n_items, ws, inv);
+                                                                          chi, gamma, cnt, only, n_items, n_dev, ws,
+                                                                          inv);
   LDGA_HIP_TRY(ctx, hipGetLastError());
   return LDGA_OK;
 }

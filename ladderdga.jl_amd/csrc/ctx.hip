@@ -297,7 +297,7 @@ static int h2d(ldga_ctx *ctx, const char *role, const void *src, size_t bytes, v
 }
 
 static int d2h(ldga_ctx *ctx, void *dst, const void *src, size_t bytes) {
-  LDGA_HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  if (bytes) LDGA_HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
   LDGA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return LDGA_OK;
 }
@@ -370,8 +370,9 @@ int ldga_chi_gamma(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, dou
   if ((rc = h2d(ctx, "chi0", chi0, cb, &c))) return rc;
   if (!(xo = ldga_buf(ctx, "chi", Nq * Nw * 8))) return LDGA_ERR_HIP;
   if (!(yo = ldga_buf(ctx, "gamma", Nq * Nnu * Nw * 16))) return LDGA_ERR_HIP;
+  ldga_bse_report local;  // the host form always reads the counters (zero pivots -> status)
   int st = launch_bse(ctx, s, precision, fr, U, beta, (const double2 *)g, (const double2 *)c, (double *)xo,
-                      (double2 *)yo, report, ctx->stream);
+                      (double2 *)yo, report ? report : &local, ctx->stream);
   if (st && st != LDGA_ERR_ZERO_PIVOT) return st;
   if ((rc = d2h(ctx, chi, xo, Nq * Nw * 8))) return rc;
   if ((rc = d2h(ctx, gamma, yo, Nq * Nnu * Nw * 16))) return rc;
@@ -430,10 +431,16 @@ int ldga_gen_chi(ldga_ctx *ctx, const ldga_freq *fr, const double *Gamma, const 
   void *g, *c, *o;
   if ((rc = h2d(ctx, "Gamma", Gamma, gb, &g))) return rc;
   if ((rc = h2d(ctx, "chi0", chi0, cb, &c))) return rc;
-  if (!(o = ldga_buf(ctx, "gen_chi", ob))) return LDGA_ERR_HIP;
-  const int st = launch_gen_chi(ctx, fr, (const double2 *)g, (const double2 *)c, (double2 *)o, ctx->stream);
-  if (st && st != LDGA_ERR_ZERO_PIVOT) return st;
-  if ((rc = d2h(ctx, inv, o, ob))) return rc;
+  // the inverses (Nnu^2 Nq Nw complex: 15.6 GB at 64^2, Nnu = 120) are staged
+  // for this call only, not cached in the context
+  LDGA_HIP_TRY(ctx, hipMalloc(&o, std::max<size_t>(ob, 1)));
+  int st = launch_gen_chi(ctx, fr, (const double2 *)g, (const double2 *)c, (double2 *)o, ctx->stream);
+  if (!st || st == LDGA_ERR_ZERO_PIVOT) {
+    rc = d2h(ctx, inv, o, ob);
+    if (rc) st = rc;
+  }
+  hipStreamSynchronize(ctx->stream);
+  hipFree(o);
   return st;
 }
 
@@ -495,6 +502,35 @@ int ldga_sum_kw_dev(ldga_ctx *ctx, const void *chi, int Nw, int w_lo, int w_hi, 
   hipSetDevice(ctx->device);
   return launch_sum_kw(ctx, (const double *)chi, Nw, w_lo, w_hi, c2, beta, lam, (double *)out_dev,
                        ldga_stream(ctx, stream));
+}
+
+int ldga_sum_w(ldga_ctx *ctx, const double *chi, int Nq, int Nw, int w_lo, int w_hi, double c2, double beta,
+               double lam, double *out_q, double *out_k) {
+  if (!ctx || !chi) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  if (Nq < 1 || Nw < 1) return ldga_fail(ctx, LDGA_ERR_SHAPE, "bad sum_ω shape Nq=%d Nw=%d", Nq, Nw);
+  hipSetDevice(ctx->device);
+  void *c, *r, *k = nullptr;
+  int rc;
+  if ((rc = h2d(ctx, "sum_chi", chi, (size_t)Nq * Nw * 8, &c))) return rc;
+  if (!(r = ldga_buf(ctx, "sum_w_res", (size_t)Nq * 8))) return LDGA_ERR_HIP;
+  if (out_k && !(k = ldga_buf(ctx, "sum_out", 3 * 8))) return LDGA_ERR_HIP;
+  if ((rc = launch_sum_w(ctx, (const double *)c, Nq, Nw, w_lo, w_hi, c2, beta, lam, (double *)r, (double *)k,
+                         ctx->stream)))
+    return rc;
+  if (out_q && (rc = d2h(ctx, out_q, r, (size_t)Nq * 8))) return rc;
+  if (out_k && (rc = d2h(ctx, out_k, k, 8))) return rc;
+  return d2h(ctx, nullptr, nullptr, 0);
+}
+
+int ldga_sum_w_dev(ldga_ctx *ctx, const void *chi, int Nq, int Nw, int w_lo, int w_hi, double c2, double beta,
+                   double lam, void *out_q, void *out_k, void *stream) {
+  if (!ctx || !chi) return ldga_fail(ctx, LDGA_ERR_ARG, "NULL argument");
+  if (Nq < 1) return ldga_fail(ctx, LDGA_ERR_SHAPE, "bad sum_ω shape Nq=%d", Nq);
+  hipSetDevice(ctx->device);
+  double *r = (double *)out_q;
+  if (!r && !(r = (double *)ldga_buf(ctx, "sum_w_res", (size_t)Nq * 8))) return LDGA_ERR_HIP;
+  return launch_sum_w(ctx, (const double *)chi, Nq, Nw, w_lo, w_hi, c2, beta, lam, r, (double *)out_k,
+                      ldga_stream(ctx, stream));
 }
 
 int ldga_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m, double lam_d,

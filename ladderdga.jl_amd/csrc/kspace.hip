@@ -289,6 +289,45 @@ __global__ __launch_bounds__(kSumNT) void sum_kw_kernel(const double *__restrict
 }
 
 // ---------------------------------------------------------------------------
+// sum_ω per q (DataTypes.jl:285-305): one thread per q walks the usable slice
+// w_lo..w_hi, pairing slice element n with the n-th frequency m = n - n_iw of
+// the whole grid for the 1/(i w_m)^2 tail, as the reference's loop does;
+// res[q] = (sum_n [chi_λ - Re(c2/(i w_m)^2)] - c2 beta^2/12) / beta.
+// Threads run over q, the fastest index of chi, so every load is coalesced.
+// ---------------------------------------------------------------------------
+__global__ void sum_w_kernel(const double *__restrict__ chi, int Nq, int w_lo, int w_hi, int n_iw, double lam,
+                             double c2, double beta, double *__restrict__ res) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= Nq) return;
+  const double tfac = -c2 * beta * beta / (4.0 * M_PI * M_PI);  // Re(c2/(i w_m)^2) = tfac / m^2
+  double r = 0.0;
+  for (int n = 0; n <= w_hi - w_lo; ++n) {
+    double x = chi[q + (size_t)Nq * (w_lo + n)];
+    if (lam != 0.0) x = x / (lam * x + 1.0);
+    const int m = n - n_iw;
+    r += (m != 0) ? x - tfac / ((double)m * (double)m) : x;
+  }
+  res[q] = (r + c2 * (-beta * beta / 12.0)) / beta;
+}
+
+// kintegrate(res) = sum_q kMult[q] res[q] / sum_q kMult[q]: one workgroup (Nq <= a few thousand)
+__global__ __launch_bounds__(kSumNT) void kintegrate_kernel(const double *__restrict__ res,
+                                                            const double *__restrict__ kmult, int Nq, double Nk,
+                                                            double *__restrict__ out) {
+  __shared__ double red[kSumNT / 64];
+  double s = 0.0;
+  for (int q = threadIdx.x; q < Nq; q += kSumNT) s = fma(kmult[q], res[q], s);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0;
+    for (int i = 0; i < kSumNT / 64; ++i) a += red[i];
+    out[0] = a / Nk;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Adjoint of the EoM with respect to (lam_m, lam_d).  Given the weight W(k,nu)
 // = dR/dSigma(k,nu) (a symmetric function on the full grid, stored on the
 // wedge), sum_k W(k) conv(K', G)(k) = sum_q K'(q) conv(W, G)(q), so one
@@ -565,6 +604,22 @@ int launch_sum_kw(ldga_ctx *ctx, const double *chi, int Nw, int w_lo, int w_hi, 
   return LDGA_OK;
 }
 
+// sum_ω per q into res (Nq doubles) and, when out_k != NULL, its k integral
+int launch_sum_w(ldga_ctx *ctx, const double *chi, int Nq, int Nw, int w_lo, int w_hi, double c2, double beta,
+                 double lam, double *res, double *out_k, hipStream_t st) {
+  if (Nq < 1 || Nw < 1 || (Nw & 1) == 0 || w_lo < 0 || w_hi >= Nw || w_lo > w_hi)
+    return ldga_fail(ctx, LDGA_ERR_SHAPE, "bad sum_ω shape: Nq=%d, omega range [%d,%d] of %d", Nq, w_lo, w_hi, Nw);
+  if (out_k && (!ctx->has_kgrid || ctx->Nq != Nq))
+    return ldga_fail(ctx, LDGA_ERR_STATE, "sum_ωk needs the k grid of the Nq=%d points of chi", Nq);
+  sum_w_kernel<<<(Nq + 255) / 256, 256, 0, st>>>(chi, Nq, w_lo, w_hi, (Nw - 1) / 2, lam, c2, beta, res);
+  LDGA_HIP_TRY(ctx, hipGetLastError());
+  if (out_k) {
+    kintegrate_kernel<<<1, kSumNT, 0, st>>>(res, ctx->oct.kmult, Nq, ctx->oct.Nk, out_k);
+    LDGA_HIP_TRY(ctx, hipGetLastError());
+  }
+  return LDGA_OK;
+}
+
 // omega chunking of the EoM: enough (nu, chunk) workgroups to fill the device
 int eom_chunks(int Nw, int numax, int *wchunk) {
   int nchunks = std::max(1, std::min(Nw, (2048 + numax - 1) / numax));
@@ -593,7 +648,7 @@ int launch_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int n
   if (!partial || !tsum) return LDGA_ERR_HIP;
   const bool add_tail = tc && eom_kind == LDGA_EOM_FULL;
   if (add_tail) {
-    int rc = launch_sum_kw(ctx, chi_m, Nw, 0, Nw - 1, mp->Ekin_DMFT, mp->beta, lam_m, tsum, st);
+    int rc = launch_sum_kw(ctx, chi_m, Nw, 0, Nw - 1, mp->chi_m_c2, mp->beta, lam_m, tsum, st);
     if (rc) return rc;
   }
   const size_t smem = (size_t)T.NO * 16 + (size_t)T.Ns * 8;

@@ -36,9 +36,9 @@ struct FinalArgs {
   const double2 *sigma;  // Nq x numax
   const double *chim, *chid;
   int Nq, numax, Nw, n_iw;
-  double Nk, beta, U, n, mu0, shell, lam_m, lam_d, c2, tol;
+  double Nk, beta, U, n, mu0, shell, lam_m, lam_d, c2m, c2d, tol;
   int fit_mu, want_jac;
-  double *res;   // 16 doubles: n mu Ekin Epot1 Epot2 lhs res_pp res_epot | Sm' Sd' Cm Fmu Emu it conv
+  double *res;   // 16 doubles: n mu Ekin Epot1 Epot2 lhs res_pp res_epot | Sm' Sd' - Fmu Emu it conv n_full
   double2 *W;    // Nq x numax adjoint weight (want_jac)
   double2 *Wk;   // numax: sum_k kMult W(k, nu)
 };
@@ -69,14 +69,14 @@ __global__ __launch_bounds__(kFT) void lambdadm_final_kernel(OctTables T, FinalA
     v4[3] = fma(km, -xd * xd, v4[3]);
   }
   block_sum<4>(v4, red);
-  double tail = 0.0;
+  // the 1/(i w_m)^2 tail over all w per unit c2 (DataTypes.jl:223-225); each sum has its own c2
+  double tail1 = 0.0;
   for (int w = 0; w < A.Nw; ++w) {
     const int m = w - A.n_iw;
-    if (m != 0) tail += -A.c2 * beta * beta / (4.0 * M_PI * M_PI * (double)m * (double)m);
+    if (m != 0) tail1 += -beta * beta / (4.0 * M_PI * M_PI * (double)m * (double)m);
   }
-  const double konst = -beta * beta / 12.0 * A.c2;
-  const double Sm = (v4[0] / Nk - tail + konst) / beta, dSm = v4[1] / Nk / beta;
-  const double Sd = (v4[2] / Nk - tail + konst) / beta, dSd = v4[3] / Nk / beta;
+  const double Sm = (v4[0] / Nk - A.c2m * tail1 - beta * beta / 12.0 * A.c2m) / beta, dSm = v4[1] / Nk / beta;
+  const double Sd = (v4[2] / Nk - A.c2d * tail1 - beta * beta / 12.0 * A.c2d) / beta, dSd = v4[3] / Nk / beta;
 
   // ---- filling_pos(G[:, 0:numax-1]) (GFTools.jl:255-273) as a function of mu
   const size_t ne = (size_t)A.Nq * A.numax;
@@ -99,6 +99,7 @@ __global__ __launch_bounds__(kFT) void lambdadm_final_kernel(OctTables T, FinalA
   if (A.fit_mu) {
     double x0 = A.mu0, x1 = (x0 == 0.0) ? 1e-4 : x0 * (1.0 + 1e-4);
     double f0 = filling(x0, sumG(x0)) - n, f1 = filling(x1, sumG(x1)) - n;
+    double fmu = f1;  // f at the returned mu
     conv = 0;
     for (iters = 0; iters < 100; ++iters) {
       if (fabs(f1) < A.tol) {
@@ -113,6 +114,7 @@ __global__ __launch_bounds__(kFT) void lambdadm_final_kernel(OctTables T, FinalA
       f1 = filling(x1, sumG(x1)) - n;
     }
     mu = x1;
+    fmu = f1;
     if (!conv) {
       // fallback, as a bracketing solver would (the filling grows with mu): step
       // out from mu0 until the sign changes, then Illinois false position
@@ -132,6 +134,7 @@ __global__ __launch_bounds__(kFT) void lambdadm_final_kernel(OctTables T, FinalA
           const double c = (a * fb - b * fa) / (fb - fa);
           const double fc = filling(c, sumG(c)) - n;
           mu = c;
+          fmu = fc;
           if (fabs(fc) < A.tol || fabs(b - a) <= 4e-16 * fmax(fabs(a), fabs(b))) {
             conv = 1;
             break;
@@ -150,6 +153,10 @@ __global__ __launch_bounds__(kFT) void lambdadm_final_kernel(OctTables T, FinalA
         }
       }
     }
+    // a fit that did not reach the reference's tolerance (find_zero atol = 1e-8,
+    // GFTools.jl:171-176) fails: mu = NaN, and every result below is NaN
+    conv = fabs(fmu) <= 1e-8;
+    if (!conv) mu = NAN;
   }
   // ---- filling, calc_E (thermodynamics.jl:142-168), and for the Jacobian
   //      d(filling)/dmu and dE_pot/dmu
@@ -196,6 +203,7 @@ __global__ __launch_bounds__(kFT) void lambdadm_final_kernel(OctTables T, FinalA
     r[9] = dSd;
     r[13] = (double)iters;
     r[14] = (double)conv;
+    r[15] = NAN;  // the full filling, written by nfull_kernel when G_ladder is formed
   }
   if (!A.want_jac) return;
   // ---- adjoint weight W = dR_epot/dSigma (full-grid function, wedge values)
@@ -235,19 +243,47 @@ __global__ void mix_kernel(double2 *__restrict__ sigma, const double2 *__restric
 }
 
 // G_ladder over fft_range: Sigma_ladder below numax, Sigma_loc above, negative
-// frequencies by conjugation (GFTools.jl:162-186)
-__global__ void gladder_kernel(OctTables T, int numax, int fft_first, int n_fft, double beta, const double *res,
-                               const double2 *__restrict__ sigma, const double2 *__restrict__ sigloc,
-                               double2 *__restrict__ G) {
+// frequencies by conjugation (GFTools.jl:162-186).  Each workgroup also leaves
+// sum kMult Re G over its elements in part[blockIdx.x], the pieces of the full
+// filling (GFTools.jl:234-241) that nfull_kernel adds up.
+constexpr int kGL = 256;
+__global__ __launch_bounds__(kGL) void gladder_kernel(OctTables T, int numax, int fft_first, int n_fft, double beta,
+                                                      const double *res, const double2 *__restrict__ sigma,
+                                                      const double2 *__restrict__ sigloc, double2 *__restrict__ G,
+                                                      double *__restrict__ part) {
+  __shared__ double red[kGL / 64];
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (size_t)T.Nq * n_fft) return;
-  const int q = (int)(i % T.Nq), col = (int)(i / T.Nq);
-  const int nun = fft_first + col;
-  const int m = nun >= 0 ? nun : -nun - 1;
-  const double2 sig = (m < numax) ? sigma[q + (size_t)T.Nq * m] : sigloc[m];
-  double2 g = green((2.0 * m + 1.0) * M_PI / beta, res[1], T.eps[q], sig);
-  if (nun < 0) g.y = -g.y;
-  G[i] = g;
+  double re = 0.0;
+  if (i < (size_t)T.Nq * n_fft) {
+    const int q = (int)(i % T.Nq), col = (int)(i / T.Nq);
+    const int nun = fft_first + col;
+    const int m = nun >= 0 ? nun : -nun - 1;
+    const double2 sig = (m < numax) ? sigma[q + (size_t)T.Nq * m] : sigloc[m];
+    double2 g = green((2.0 * m + 1.0) * M_PI / beta, res[1], T.eps[q], sig);
+    if (nun < 0) g.y = -g.y;
+    G[i] = g;
+    re = T.kmult[q] * g.x;
+  }
+  for (int off = 32; off > 0; off >>= 1) re += __shfl_xor(re, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = re;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// filling(G_ladder, kG, U, mu, beta) = 2 (Re sum_nu kintegrate(G)/beta + 1/2 + mu shell)/(1 + U shell),
+// shell = G_shell_sum(floor(n_fft / 2), beta)  (GFTools.jl:234-241), into res[15]
+__global__ __launch_bounds__(kGL) void nfull_kernel(const double *__restrict__ part, int nparts, double Nk,
+                                                    double beta, double U, double shell, double *res) {
+  __shared__ double red[kGL / 64];
+  double s = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += kGL) s += part[k];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double reSG = (red[0] + red[1] + red[2] + red[3]) / Nk;
+    res[15] = 2.0 * (reSG / beta + 0.5 + res[1] * shell) / (1.0 + U * shell);
+  }
 }
 
 // trigamma(x) for x > 0: recurrence up to x >= 20, then the asymptotic series
@@ -289,6 +325,7 @@ static int lambdadm_dev(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp
   if (lam_m > 1e5 || lam_d > 1e5) {  // conditions_new.jl:251
     const double nan = std::nan("");
     out->n = out->mu = out->Ekin = out->Epot1 = out->Epot2 = out->lhs_c1 = out->res_pp = out->res_epot = nan;
+    out->n_full = nan;
     for (double &j : out->jac) j = nan;
     return LDGA_OK;
   }
@@ -321,7 +358,8 @@ static int lambdadm_dev(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp
   A.shell = ldga_G_shell_sum(numax + 1, mp->beta);
   A.lam_m = lam_m;
   A.lam_d = lam_d;
-  A.c2 = mp->Ekin_DMFT;
+  A.c2m = mp->chi_m_c2;
+  A.c2d = mp->chi_d_c2;
   A.tol = 1e-14;
   A.fit_mu = fit_mu;
   A.want_jac = want_jac;
@@ -340,8 +378,14 @@ static int lambdadm_dev(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp
   }
   if (G_out) {
     const size_t tot = (size_t)Nq * fr->n_fft;
-    gladder_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(ctx->oct, numax, fr->fft_first, fr->n_fft, mp->beta,
-                                                                res, sigma, sigloc, G_out);
+    const unsigned nb = (unsigned)((tot + kGL - 1) / kGL);
+    double *part = (double *)ldga_buf(ctx, "ldm_gl_part", (size_t)nb * 8);
+    if (!part) return LDGA_ERR_HIP;
+    gladder_kernel<<<nb, kGL, 0, st>>>(ctx->oct, numax, fr->fft_first, fr->n_fft, mp->beta, res, sigma, sigloc, G_out,
+                                       part);
+    LDGA_HIP_TRY(ctx, hipGetLastError());
+    nfull_kernel<<<1, kGL, 0, st>>>(part, (int)nb, ctx->oct.Nk, mp->beta, mp->U,
+                                    ldga_G_shell_sum(fr->n_fft / 2, mp->beta), res);
     LDGA_HIP_TRY(ctx, hipGetLastError());
   }
   double h[20];
@@ -362,6 +406,7 @@ static int lambdadm_dev(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp
   out->res_epot = h[7];
   out->mu_iters = (int32_t)h[13];
   out->mu_converged = (int32_t)h[14];
+  out->n_full = h[15];
   if (want_jac) {
     const double dSm = h[8], dSd = h[9], U = mp->U, beta = mp->beta;
     // d res_pp / d lam = (1/2) dΣχ/dlam

@@ -106,6 +106,8 @@ int launch_F_from_chi(ldga_ctx *ctx, const ldga_freq *fr, double beta, int diag_
                       const double2 *G, double2 *F, hipStream_t st);
 int launch_gen_chi(ldga_ctx *ctx, const ldga_freq *fr, const double2 *Gam, const double2 *chi0, double2 *inv,
                    hipStream_t st);
+int launch_sum_w(ldga_ctx *ctx, const double *chi, int Nq, int Nw, int w_lo, int w_hi, double c2, double beta,
+                 double lam, double *res, double *out_k, hipStream_t st);
 // EoM partial sums: one (nu, omega-chunk) workgroup each; returns the chunking used
 int eom_chunks(int Nw, int numax, int *wchunk);
 int launch_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m, double lam_d,

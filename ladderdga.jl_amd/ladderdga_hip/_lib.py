@@ -32,7 +32,7 @@ EXPORTED = [
     "ldga_g_tables", "ldga_g_tables_dev", "ldga_bubble", "ldga_bubble_dev",
     "ldga_chi_gamma", "ldga_chi_gamma_dev",
     "ldga_lambda0", "ldga_lambda0_dev", "ldga_F_from_chi", "ldga_F_from_chi_dev", "ldga_gen_chi", "ldga_gen_chi_dev",
-    "ldga_sum_kw", "ldga_sum_kw_dev",
+    "ldga_sum_kw", "ldga_sum_kw_dev", "ldga_sum_w", "ldga_sum_w_dev",
     "ldga_sigma", "ldga_sigma_dev",
     "ldga_malloc", "ldga_free", "ldga_memcpy_h2d", "ldga_memcpy_d2h",
     "ldga_G_shell_sum", "ldga_lambdadm_residual", "ldga_lambdadm_residual_dev", "ldga_g_from_sigma_ladder_dev",
@@ -46,7 +46,7 @@ class Freq(C.Structure):
 
 class Model(C.Structure):
     _fields_ = [("U", C.c_double), ("mu", C.c_double), ("beta", C.c_double), ("n", C.c_double),
-                ("Ekin_DMFT", C.c_double)]
+                ("Ekin_DMFT", C.c_double), ("chi_m_c2", C.c_double), ("chi_d_c2", C.c_double)]
 
 
 class BseReport(C.Structure):
@@ -58,7 +58,8 @@ class BseReport(C.Structure):
 class LambdaDmOut(C.Structure):
     _fields_ = [("n", C.c_double), ("mu", C.c_double), ("Ekin", C.c_double), ("Epot1", C.c_double),
                 ("Epot2", C.c_double), ("lhs_c1", C.c_double), ("res_pp", C.c_double), ("res_epot", C.c_double),
-                ("jac", C.c_double * 4), ("mu_iters", C.c_int32), ("mu_converged", C.c_int32)]
+                ("jac", C.c_double * 4), ("mu_iters", C.c_int32), ("mu_converged", C.c_int32),
+                ("n_full", C.c_double)]
 
 
 class LdgaError(RuntimeError):
@@ -102,6 +103,8 @@ def lib():
             "ldga_lambda0_dev": (i32, [vp, C.POINTER(Freq), dbl, vp, vp, vp, vp]),
             "ldga_sum_kw": (i32, [vp, vp, i32, i32, i32, dbl, dbl, dbl, vp]),
             "ldga_sum_kw_dev": (i32, [vp, vp, i32, i32, i32, dbl, dbl, dbl, vp, vp]),
+            "ldga_sum_w": (i32, [vp, vp, i32, i32, i32, i32, dbl, dbl, dbl, vp, vp]),
+            "ldga_sum_w_dev": (i32, [vp, vp, i32, i32, i32, i32, dbl, dbl, dbl, vp, vp, vp]),
             "ldga_sigma": (i32, [vp, C.POINTER(Freq), C.POINTER(Model), i32, dbl, dbl, i32, i32, dbl,
                                  vp, vp, vp, vp, vp, vp, vp]),
             "ldga_sigma_dev": (i32, [vp, C.POINTER(Freq), C.POINTER(Model), i32, dbl, dbl, i32, i32, dbl,

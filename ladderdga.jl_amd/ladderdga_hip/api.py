@@ -24,8 +24,17 @@ def _freq(sP: SimulationParameters) -> Freq:
     return Freq(sP.n_iω, sP.n_iν, sP.n_iν_shell, int(sP.shift), sP.fft_range.start, len(sP.fft_range))
 
 
-def _model(mP: ModelParameters) -> Model:
-    return Model(mP.U, mP.μ, mP.β, mP.n, mP.Ekin_DMFT)
+def _tail_c2(χ, default: float) -> float:
+    """c₂ of χ's 1/ω² tail (χ.tail_c[3] in Julia's numbering); 0 for an empty tail, as sum_kω takes it."""
+    if χ is None:
+        return default
+    return float(χ.tail_c[2]) if len(χ.tail_c) == 3 else 0.0
+
+
+def _model(mP: ModelParameters, χm=None, χd=None) -> Model:
+    """The model parameters and the tail coefficients of the χ sums: χm.tail_c[3] and χd.tail_c[3]
+    (Ekin_DMFT when a χ is not given, the value calc_χγ sets)."""
+    return Model(mP.U, mP.μ, mP.β, mP.n, mP.Ekin_DMFT, _tail_c2(χm, mP.Ekin_DMFT), _tail_c2(χd, mP.Ekin_DMFT))
 
 
 def _ctx(ctx, kG):
@@ -220,13 +229,42 @@ def sum_kω(kG: KGrid, χ: χT, λ: float = math.nan, force_full_range=False, ct
     return (out[0], out[2]) if with_derivative else float(out[0])
 
 
+def _sum_w_call(ctx, χ: χT, lo: int, hi: int, lam: float, want_q: bool, want_k: bool):
+    c = χ.tail_c
+    if len(c) != 3:
+        raise ValueError("sum_ω needs the three tail coefficients χ.tail_c = [c₀, c₁, c₂]")
+    if not math.isclose(sum(c), c[2], rel_tol=1.4901161193847656e-08):
+        raise ValueError("sum_ω only implemented for 1/ω^2, but other tail coefficients are non-zero!")
+    Nq, Nω = χ.data.shape
+    out_q = np.empty(Nq) if want_q else None
+    out_k = np.zeros(1) if want_k else None
+    ctx.check(_lib.lib().ldga_sum_w(ctx.handle, ptr(χ.data, f64, "χ.data"), Nq, Nω, lo, hi, float(c[2]), χ.β,
+                                    lam, out_q.ctypes.data if want_q else None,
+                                    out_k.ctypes.data if want_k else None))
+    return out_q, out_k
+
+
+def sum_ω(χ: χT, force_full_range=False, ctx=None) -> np.ndarray:
+    """sum_ω(χ) (src/DataTypes.jl:277-308): per q, Σ_ω over the usable range with the c₂/ω² tail
+    subtracted and its exact sum -c₂β²/12 added, divided by β.  As in the reference, element n of the
+    usable slice is paired with the n-th frequency of the whole ω grid for the tail term
+    (DataTypes.jl:293-301); on the full range that is its own frequency."""
+    ctx = ctx or default_context()
+    lo, hi = (0, χ.data.shape[1] - 1) if force_full_range else _usable(χ)
+    return _sum_w_call(ctx, χ, lo, hi, 0.0, True, False)[0]
+
+
 def sum_ωk(kG: KGrid, χ: χT, λ: float = math.nan, force_full_range=False, ctx=None) -> float:
-    """kintegrate(sum_ω(χ)) (src/DataTypes.jl:257-264): the ω sum per q first, then the q sum.  The
-    two sums are finite and linear, so this is sum_kω in the other order; it runs the same GPU
-    reduction."""
-    if χ.λ != 0 and not math.isnan(λ) and λ != 0:
+    """sum_ωk (src/DataTypes.jl:257-264): kintegrate(sum_ω(χ_λ(χ, λ))), the ω sum per q first, on the
+    usable range (a strict subset included, see sum_ω).  χ is not modified."""
+    λ_check = not math.isnan(λ) and λ != 0
+    if χ.λ != 0 and λ_check:
         raise RuntimeError("χ already λ-corrected, but external λ provided!")
-    return sum_kω(kG, χ, λ=λ, force_full_range=force_full_range, ctx=ctx)
+    ctx = _ctx(ctx, kG)
+    if χ.data.shape[0] != kG.Nq:
+        raise ValueError(f"χ must have the Nq = {kG.Nq} rows of the k grid")
+    lo, hi = (0, χ.data.shape[1] - 1) if force_full_range else _usable(χ)
+    return float(_sum_w_call(ctx, χ, lo, hi, λ if λ_check else 0.0, False, True)[1][0])
 
 
 def Σ_loc_correction(Σ_ladder, Σ_ladderLoc, Σ_loc):
@@ -280,7 +318,7 @@ def get_λ_min(χr: np.ndarray) -> float:
 # ---------------------------------------------------------------------------
 def _sigma_call(ctx, sP, mP, νmax, λm, λd, tc, kind, χ_m_sum, χm, γm, χd, γd, λ0, Gνω_r, Nq):
     out = np.empty((Nq, νmax), dtype=c128, order="F")
-    ctx.check(_lib.lib().ldga_sigma(ctx.handle, C.byref(_freq(sP)), C.byref(_model(mP)), νmax, λm, λd, int(tc),
+    ctx.check(_lib.lib().ldga_sigma(ctx.handle, C.byref(_freq(sP)), C.byref(_model(mP, χm, χd)), νmax, λm, λd, int(tc),
                                     kind, χ_m_sum, ptr(χm.data, f64, "χm"), ptr(γm.data, c128, "γm"),
                                     ptr(χd.data, f64, "χd"), ptr(γd.data, c128, "γd"), ptr(λ0, c128, "λ0"),
                                     ptr(Gνω_r, c128, "Gνω_r"), out.ctypes.data))

@@ -15,7 +15,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import LambdaDmOut, default_context
-from .api import _freq, _model, get_λ_min, sum_kω
+from .api import _freq, _model, get_λ_min, sum_kω, sum_ωk
 from .kgrid import KGrid
 from .types import ModelParameters, SimulationParameters, χT, γT
 
@@ -199,7 +199,7 @@ class LambdaDmProblem:
                       γm=up(np.asarray(γm.data, np.complex128)), γd=up(np.asarray(γd.data, np.complex128)),
                       λ0=up(np.asarray(λ0, np.complex128)), G=up(np.asarray(gLoc_rfft, np.complex128)),
                       Σloc=up(np.ascontiguousarray(Σ_loc[:sP.fft_range.stop], np.complex128)))
-        self.fr, self.mo = _freq(sP), _model(mP)
+        self.fr, self.mo = _freq(sP), _model(mP, χm, χd)
         self.μ = mP.μ if μ is None else float(μ)      # start of the first μ fit
         self.evals = 0
 
@@ -220,7 +220,7 @@ class LambdaDmProblem:
         if fit_μ and math.isfinite(out.mu):
             self.μ = out.mu          # the reference's closure keeps the last μ as the next start
         res = {k: getattr(out, k) for k in ("n", "mu", "Ekin", "Epot1", "Epot2", "lhs_c1", "res_pp", "res_epot",
-                                            "mu_iters", "mu_converged")}
+                                            "mu_iters", "mu_converged", "n_full")}
         res["residual"] = np.array([out.res_pp, out.res_epot])
         res["jac"] = np.array(out.jac[:]).reshape(2, 2) if want_jac else None
         if arrays:
@@ -350,7 +350,7 @@ def run_sc(χm: χT, γm: γT, χd: χT, γd: γT, λ0, μ: float, Σ_loc, gLoc_
     d["Gf"] = ctx.empty(d["Gr"].shape, np.complex128)                  # G_fft of the rebuild (unused)
     d["G"] = ctx.empty((kG.Nq, nf), np.complex128)
     cur, nxt = ctx.empty((kG.Nq, νmax), np.complex128), ctx.empty((kG.Nq, νmax), np.complex128)
-    fr, mo = _freq(sP), _model(mP)
+    fr, mo = _freq(sP), _model(mP, χm, χd)
     out = LambdaDmOut()
     # the sums at (λm, λd) before the loop (run_sc! :417-421)
     χm_sum = sum_kω(kG, χm, λ=λm, ctx=ctx)
@@ -389,12 +389,13 @@ def run_sc(χm: χT, γm: γT, χd: χT, γd: γT, λ0, μ: float, Σ_loc, gLoc_
             if type in ("m", "pre_m", "dm", "pre_dm"):
                 lhs_c1, E_pot_2 = out.lhs_c1, out.Epot2
             if trace:
-                # the reference's trace row (conditions_new.jl:507-515): χ sums in both orders and the
-                # checksums |Σ Σ_ladder|, |Σ G_ladder| of this iteration
+                # the reference's trace row (conditions_new.jl:507-515): χ sums in both orders, the
+                # filling of the whole G_ladder and the checksums |Σ Σ_ladder|, |Σ G_ladder|
                 cs_m, cs_d = sum_kω(kG, χm, λ=λm, ctx=ctx), sum_kω(kG, χd, λ=λd, ctx=ctx)
-                rows.append(dict(it=it, λm=λm, λd=λd, μ=μ, n=n, EKin=E_kin, EPot=E_pot_1,
-                                 lhs_c1=(cs_d + cs_m) / 2, EPot_c2=E_pot_2, cs_m=cs_m, cs_m2=cs_m, cs_d=cs_d,
-                                 cs_d2=cs_d, cs_Σ=float(abs(cur.to_host().sum())),
+                cs_m2, cs_d2 = sum_ωk(kG, χm, λ=λm, ctx=ctx), sum_ωk(kG, χd, λ=λd, ctx=ctx)
+                rows.append(dict(it=it, λm=λm, λd=λd, μ=μ, n=out.n_full, EKin=E_kin, EPot=E_pot_1,
+                                 lhs_c1=(cs_d + cs_m) / 2, EPot_c2=E_pot_2, cs_m=cs_m, cs_m2=cs_m2, cs_d=cs_d,
+                                 cs_d2=cs_d2, cs_Σ=float(abs(cur.to_host().sum())),
                                  cs_G=float(abs(d["G"].to_host().sum()))))
             if abs(E_pot_1 - E_pot_1_old) < conv_abs and abs(E_pot_2 - E_pot_2_old) < conv_abs:
                 converged = True

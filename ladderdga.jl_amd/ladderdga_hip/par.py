@@ -252,7 +252,8 @@ def calc_Σ_par_inplace(Σ_ladder, λm: float = 0.0, λd: float = 0.0, collect_d
     kG, mP, sP = wcache.kG, wcache.mP, wcache.sP
     νmax = len(wcache.EoM_νGrid)
     g = wcache.get
-    ctx.check(_lib.lib().ldga_sigma_dev(ctx.handle, C.byref(_freq(sP)), C.byref(_model(mP)), νmax, λm, λd,
+    mo = _model(mP, wcache.χ_meta["m"], wcache.χ_meta["d"])
+    ctx.check(_lib.lib().ldga_sigma_dev(ctx.handle, C.byref(_freq(sP)), C.byref(mo), νmax, λm, λd,
                                         int(tc), 0, wcache.χloc_m_sum, g("χm").ptr, g("γm").ptr, g("χd").ptr,
                                         g("γd").ptr, g("λ₀").ptr, g("G_fft_reverse").ptr, g("Σ_ladder").ptr,
                                         None))

@@ -462,30 +462,36 @@ def sum_kω_raw(kG, χ, beta, c2, ωn2_tail, transform=None):
 
 
 def sum_ω(χ: ChiT, force_full_range=False):
-    """src/DataTypes.jl:277-308."""
+    """src/DataTypes.jl:277-308, as written.  sum_ω! (:285-291) hands the inner
+    sum_ω (:293-305) the whole ωn_arr together with the usable slice of χ, and
+    the inner loop pairs slice element n with ωn_arr[n]: the n-th frequency of
+    the whole grid, counted from -n_iω, not the slice's own frequency.  On the
+    full range the two coincide."""
     ω_slice = list(range(χ.data.shape[1]))
     if not force_full_range:
         ω_slice = [ω_slice[i] for i in χ.usable_ω]
-    ωn = ωn_grid(χ)[ω_slice]
+    ωn_arr = ωn_grid(χ)
     c = χ.tail_c
+    if not math.isclose(sum(c), c[2], rel_tol=1.4901161193847656e-08):
+        raise ValueError("sum_ω only implemented for 1/ω^2, but other tail coefficients are non-zero!")
     out = np.empty(χ.data.shape[0])
     for qi in range(χ.data.shape[0]):
         res = 0.0
         for n, wi in enumerate(ω_slice):
-            if ωn[n] != 0:
-                res += χ.data[qi, wi] - np.real(c[2] / ωn[n] ** 2)
+            if ωn_arr[n] != 0:
+                res += χ.data[qi, wi] - np.real(c[2] / ωn_arr[n] ** 2)
             else:
                 res += χ.data[qi, wi]
         out[qi] = (res + c[2] * (-χ.beta ** 2 / 12)) / χ.beta
     return out
 
 
-def sum_ωk(kG, χ: ChiT, λ=None):
-    """src/DataTypes.jl:257-264."""
+def sum_ωk(kG, χ: ChiT, λ=None, force_full_range=False):
+    """src/DataTypes.jl:257-264: kintegrate(sum_ω(χ_λ(χ, λ)))."""
     if λ is not None and λ != 0:
         χ2 = ChiT(χ_λ(χ.data, λ), χ.beta, tail_c=χ.tail_c, usable_ω=χ.usable_ω)
-        return float(kG.kintegrate(sum_ω(χ2)))
-    return float(kG.kintegrate(sum_ω(χ)))
+        return float(kG.kintegrate(sum_ω(χ2, force_full_range=force_full_range)))
+    return float(kG.kintegrate(sum_ω(χ, force_full_range=force_full_range)))
 
 
 def get_λ_min(χr):
@@ -517,8 +523,10 @@ def conv_eom(kG: KGrid, K_wedge, g_rfft_slice):
     return conv_fft(kG, Kf, g_rfft_slice)
 
 
-def calc_Σ_ω(eomf, χm, γm, χd, γd, g_rfft, λ0, U, kG, sP, νmax, per_ω=False):
-    """src/ladderDGATools_singleCore.jl:28-69: the omega/nu/q loop nest (chi already lambda-mapped)."""
+def calc_Σ_ω(eomf, χm, γm, χd, γd, g_rfft, λ0, U, kG, sP, νmax, per_ω=False, νsel=None):
+    """src/ladderDGATools_singleCore.jl:28-69: the omega/nu/q loop nest (chi already lambda-mapped).
+    νsel (optional): only these ν columns (0-based ν_n) are summed, the others stay 0 -- the same
+    loop on a subset, for grids where the whole nest takes too long."""
     Nq, Nω = χm.shape
     f0 = sP.fft_range.start
     Σ = np.zeros((Nq, νmax, Nω) if per_ω else (Nq, νmax), dtype=complex)
@@ -527,6 +535,8 @@ def calc_Σ_ω(eomf, χm, γm, χd, γd, g_rfft, λ0, U, kG, sP, νmax, per_ω=F
         νZero = ν0Index_of_ωIndex(ωi + 1, sP)
         νlist = list(range(νZero, 2 * sP.n_inu + 1))[:νmax]
         for νii, νi in enumerate(νlist):
+            if νsel is not None and νii not in νsel:
+                continue
             K = eomf(U, γm[:, νi - 1, ωi], γd[:, νi - 1, ωi], χm[:, ωi], χd[:, ωi], λ0[:, νi - 1, ωi])
             c = conv_eom(kG, K, g_rfft[..., νii + ωn - f0])
             if per_ω:
@@ -537,13 +547,13 @@ def calc_Σ_ω(eomf, χm, γm, χd, γd, g_rfft, λ0, U, kG, sP, νmax, per_ω=F
 
 
 def calc_Σ(χm: ChiT, γm, χd: ChiT, γd, χ_m_sum, λ0, g_rfft, kG, mP, sP,
-           νmax=None, λm=0.0, λd=0.0, tc=True):
+           νmax=None, λm=0.0, λd=0.0, tc=True, νsel=None):
     """src/ladderDGATools_singleCore.jl:114-136 with calc_Σ! (:71-81).  Returns Sigma[q, nu] (nu = 0..numax-1)."""
     if νmax is None:
         νmax = sP.n_inu
     cm = χ_λ(χm.data, λm) if λm != 0 else χm.data
     cd = χ_λ(χd.data, λd) if λd != 0 else χd.data
-    Σ = calc_Σ_ω(EOM["full"], cm, γm, cd, γd, g_rfft, λ0, mP.U, kG, sP, νmax)
+    Σ = calc_Σ_ω(EOM["full"], cm, γm, cd, γd, g_rfft, λ0, mP.U, kG, sP, νmax, νsel=νsel)
     χm_l = ChiT(cm, χm.beta, tail_c=χm.tail_c, usable_ω=χm.usable_ω)
     tail = (-mP.U * (sum_kω(kG, χm_l) - χ_m_sum)) if tc else 0.0
     tail_c = tail / iν_array(mP.beta, range(νmax))
@@ -576,7 +586,8 @@ def G_from_Σladder(Σ_ladder, Σ_loc, kG, mP, sP, fix_n=True, μ=None, secant_t
     """src/GFTools.jl:162-186.  Returns (mu, G[q, nu in fft_range]).
 
     The mu root is found with a secant iteration to |f| < secant_tol instead
-    of Roots.find_zero(Order0, atol=1e-8); both converge to the same root.
+    of Roots.find_zero(Order0, atol=1e-8); both converge to the same root.  A
+    fit that ends with |f| > 1e-8 gives mu = NaN, as the reference's does.
     """
     if μ is None:
         μ = mP.mu
@@ -598,6 +609,9 @@ def G_from_Σladder(Σ_ladder, Σ_loc, kG, mP, sP, fix_n=True, μ=None, secant_t
 
     if fix_n:
         μ = secant(fμ, μ, tol=secant_tol)
+        # find_zero (atol 1e-8) throwing makes G_from_Σladder! return NaN (GFTools.jl:171-176)
+        if not abs(fμ(μ)) <= 1e-8:
+            μ = math.nan
     Gp = Gpos(μ)
     nf = len(sP.fft_range)
     G = np.empty((kG.Nq, nf), dtype=complex)

@@ -222,6 +222,24 @@ def test_mu_root_fallback():
     assert O.secant(lambda x: x ** 3 + x - 2.0, 0.9) == pytest.approx(1.0, abs=1e-12)
 
 
+def test_mu_fit_failure_gives_nan():
+    """G_from_Σladder! returns NaN when find_zero fails (GFTools.jl:171-176): a Σ_ladder holding a NaN
+    makes the filling NaN for every μ, and the fit reports μ = NaN; the same Σ without the NaN fits n."""
+    kG = O.KGrid(2, 4, 1.0)
+    sP_ = O.SimulationParameters(3, 4, 0, 1)
+    mP = O.ModelParameters(1.0, 0.3, 3.0, 0.9)
+    last = sP_.fft_range.stop - 1
+    Σ_loc = 0.45 + 0.25 / (4j * (2 * np.arange(last + 1) + 1) * math.pi / mP.beta)
+    Σ = np.tile(Σ_loc[:3], (kG.Nq, 1))
+    μ, _ = O.G_from_Σladder(Σ, Σ_loc, kG, mP, sP_, fix_n=True)
+    assert math.isfinite(μ)
+    G = O.G_from_Σladder(Σ, Σ_loc, kG, mP, sP_, fix_n=False, μ=μ)[1]
+    f0 = sP_.fft_range.start
+    assert O.filling_pos_kG(G[:, -f0:-f0 + 3], kG, mP.U, μ, mP.beta) == pytest.approx(mP.n, abs=1e-12)
+    Σ[1, 2] = np.nan
+    assert math.isnan(O.G_from_Σladder(Σ, Σ_loc, kG, mP, sP_, fix_n=True)[0])
+
+
 def test_bubble_local_identity():
     """kintegrate_q chi0(q,nu,w) = -beta G_loc(nu) G_loc(nu+w) (normalisation of the convolution)."""
     kG, sP_, mP, Σ_loc, gLoc, gf, gr, _ = _tiny_inputs(Ns=6)
@@ -339,3 +357,59 @@ def test_F_from_χ_closed_forms(shift):
                                                    * Gf[ωn + νpn - first])
     assert np.max(np.abs(O.F_from_χ(χ0, (Gf, first), sP, β))) <= 1e-12
     assert np.allclose(O.F_from_χ(χc, (Gf, first), sP, β, diag_term=False), -c, rtol=1e-12, atol=1e-12)
+
+
+# --------------------------------------------------------------------------- λ₀ (BSETools.jl:128-141)
+@pytest.mark.parametrize("shell", [0, 2])
+def test_calc_λ0_closed_forms(shell):
+    """λ₀[q,ν',ω] = χ₀[q, shell+ν', ω]/β² Σ_ν F[ν,ν',ω] on F whose ν-sums are known:
+    F ≡ c gives c Nν χ₀/β²; F diagonal gives F[ν',ν',ω] χ₀/β²; F nonzero only in row ν = 0 (not symmetric)
+    gives F[0,ν',ω] χ₀/β², so a sum over the second index fails.  With shell > 0 the χ₀ columns of the
+    shell hold other values, so an ignored or doubled shell offset fails too; a missing 1/β² fails all."""
+    rng = np.random.default_rng(17 + shell)
+    Nq, Nν, Nω, β = 5, 6, 3, 2.7
+    χ0 = rng.standard_normal((Nq, Nν + 2 * shell, Nω)) + 1j * rng.standard_normal((Nq, Nν + 2 * shell, Nω))
+    core = χ0[:, shell:shell + Nν, :]
+    mP = O.ModelParameters(1.0, 0.0, β, 1.0)
+    sP_ = O.SimulationParameters(1, Nν // 2, shell, 1)
+    c = 0.3 - 1.1j
+    l0 = O.calc_λ0(χ0, np.full((Nν, Nν, Nω), c), mP, sP_)
+    assert np.allclose(l0, c * Nν * core / β ** 2, rtol=1e-13, atol=0)
+    d = rng.standard_normal((Nν, Nω)) + 1j * rng.standard_normal((Nν, Nω))
+    Fd = np.zeros((Nν, Nν, Nω), complex)
+    Fd[np.arange(Nν), np.arange(Nν), :] = d
+    assert np.allclose(O.calc_λ0(χ0, Fd, mP, sP_), d[None, :, :] * core / β ** 2, rtol=1e-13, atol=0)
+    r = rng.standard_normal((Nν, Nω)) + 1j * rng.standard_normal((Nν, Nω))
+    Fr = np.zeros((Nν, Nν, Nω), complex)
+    Fr[0, :, :] = r
+    assert np.allclose(O.calc_λ0(χ0, Fr, mP, sP_), r[None, :, :] * core / β ** 2, rtol=1e-13, atol=0)
+
+
+# --------------------------------------------------------------------------- sum_ω on a usable slice
+def test_sum_ω_strict_usable_range_known_answer():
+    """sum_ω (DataTypes.jl:285-305) on a strict usable range, evaluated by hand: β = 2π puts ω_m = i m, so
+    Re(c₂/ω_m²) = -c₂/m².  Nω = 5 (ω_m, m = -2..2), usable indices 2:4 (1-based), χ[q, 2:4] = (1, 2, 3),
+    c₂ = 1.  The reference pairs slice element n with ωn_arr[n] (m = -2, -1, 0), so the tail terms
+    subtracted are -1/4, -1 and none: ((1 + 2 + 3) + 1/4 + 1 - β²/12)/β = (7.25 - π²/3)/(2π).  Pairing
+    each element with its own frequency (m = -1, 0, 1) would give (8 - π²/3)/(2π) instead."""
+    β = 2 * math.pi
+    d = np.array([[9.0, 1.0, 2.0, 3.0, -7.0], [9.0, 1.0, 2.0, 3.0, -7.0]])
+    χ = O.ChiT(d, β, tail_c=[0.0, 0.0, 1.0], usable_ω=range(1, 4))
+    want = (7.25 - math.pi ** 2 / 3) / (2 * math.pi)
+    assert O.sum_ω(χ) == pytest.approx([want, want], rel=1e-15)
+    assert abs(want - (8 - math.pi ** 2 / 3) / (2 * math.pi)) > 0.1
+    # force_full_range: all five, each with its own frequency: (8 + 2(1/4 + 1) - π²/3)/(2π)
+    assert O.sum_ω(χ, force_full_range=True)[0] == pytest.approx((8 + 2.5 - math.pi ** 2 / 3) / (2 * math.pi),
+                                                                  rel=1e-15)
+    # sum_ωk = kintegrate of the per-q sums; the same value on every q integrates to itself
+    kG = O.gen_kgrid("2Dsc-1.0", 2)
+    d3 = np.tile(d[0], (kG.Nq, 1))
+    assert O.sum_ωk(kG, O.ChiT(d3, β, tail_c=[0.0, 0.0, 1.0], usable_ω=range(1, 4))) == pytest.approx(want,
+                                                                                                    rel=1e-15)
+    # λ maps χ before the sums: χ_λ = χ/(λχ + 1) on the slice values
+    lam = 0.5
+    wl = ((1 / 1.5 + 2 / 2 + 3 / 2.5) + 1.25 - math.pi ** 2 / 3) / (2 * math.pi)
+    assert O.sum_ωk(kG, O.ChiT(d3, β, tail_c=[0.0, 0.0, 1.0], usable_ω=range(1, 4)), λ=lam) == pytest.approx(
+        wl, rel=1e-14)
+    with pytest.raises(ValueError):
+        O.sum_ω(O.ChiT(d, β, tail_c=[0.0, 1.0, 1.0]))
