@@ -142,6 +142,21 @@ int ldga_chi_gamma_dev(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr,
                        const void *Gamma, const void *chi0, void *chi, void *gamma, ldga_bse_report *report,
                        void *stream);
 
+/* ---- allocation-free BSE ---------------------------------------------------
+ * No counterpart in the reference.  ldga_chi_gamma_dev without any device
+ * allocation, for callers that cannot allocate: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  Same kernels and
+ * results as ldga_chi_gamma_dev (FP64: the blocked kernel, whatever
+ * LDGA_BSE_F64 says); its scratch (report counters, the mixed re-solve list)
+ * is a static device array of 4 MiB + 256 bytes per GPU, held until the process
+ * ends.  Limits: Nnu <= 128 and Nq (2 n_iw + 1) <= 1048576 matrices, else
+ * LDGA_ERR_SHAPE.  All calls on one GPU share the array: calls that overlap on
+ * one GPU race.  Without a report the call is asynchronous and makes no host
+ * synchronisation. */
+int ldga_chi_gamma_noalloc_dev(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double U, double beta,
+                               const void *Gamma, const void *chi0, void *chi, void *gamma, ldga_bse_report *report,
+                               void *stream);
+
 /* ---- generalized susceptibility: calc_gen_χ (src/BSETools.jl:259-275) --
  * inv[:,:,q,w] = (Gamma[:,:,w] + diag(1/chi0[q, shell+1:shell+Nnu, w]))^-1,
  * inv (Nnu x Nnu x Nq x Nw) complex, in FP64 by Gauss-Jordan with column

@@ -333,22 +333,33 @@ static bool use_register_gj() {
 
 int launch_bse(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double U, double beta, const double2 *Gam,
                const double2 *chi0, double *chi, double2 *gamma, ldga_bse_report *rep, hipStream_t st) {
-  if (s != 1 && s != -1) return ldga_fail(ctx, LDGA_ERR_ARG, "channel sign s must be +1 (m) or -1 (d)");
-  const int N = 2 * fr->n_inu, Nw = 2 * fr->n_iw + 1, Nq = ctx->Nq, Nc = N + 2 * fr->n_inu_shell;
+  const int Nw = 2 * fr->n_iw + 1, Nq = ctx->Nq;
   // counters: [0] zero pivots, [1] refinement steps max, [2] unconverged refinements,
   // [3] largest final refinement ratio (bits), [5] refinement steps summed over the
   // matrices; [8..15]: the same for the FP64 re-solve
   unsigned long long *cnt = (unsigned long long *)ldga_buf(ctx, "bse_counters", 16 * 8);
   if (!cnt) return LDGA_ERR_HIP;
-  bse_clear_counters<<<1, 16, 0, st>>>(cnt, 16);
-  LDGA_HIP_TRY(ctx, hipGetLastError());
   // mixed mode: the refinement's failures are listed (room for every matrix)
   // and re-solved in FP64
-  const bool resolve = precision == LDGA_PREC_MIXED;
   int *redo = nullptr;
-  if (resolve && !(redo = (int *)ldga_buf(ctx, "bse_redo", (size_t)Nq * Nw * sizeof(int)))) return LDGA_ERR_HIP;
+  if (precision == LDGA_PREC_MIXED && !(redo = (int *)ldga_buf(ctx, "bse_redo", (size_t)Nq * Nw * sizeof(int))))
+    return LDGA_ERR_HIP;
+  return launch_bse_with(ctx, s, precision, fr, U, beta, Gam, chi0, chi, gamma, rep, st, cnt, redo, use_register_gj());
+}
+
+// The BSE with caller-provided scratch: cnt (16 counters) and, in mixed mode,
+// redo (room for Nq Nw ints).  register_gj selects the register-tile FP64
+// kernel instead of the blocked one.
+int launch_bse_with(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double U, double beta,
+                    const double2 *Gam, const double2 *chi0, double *chi, double2 *gamma, ldga_bse_report *rep,
+                    hipStream_t st, unsigned long long *cnt, int *redo, bool register_gj) {
+  if (s != 1 && s != -1) return ldga_fail(ctx, LDGA_ERR_ARG, "channel sign s must be +1 (m) or -1 (d)");
+  const int N = 2 * fr->n_inu, Nw = 2 * fr->n_iw + 1, Nq = ctx->Nq, Nc = N + 2 * fr->n_inu_shell;
+  const bool resolve = precision == LDGA_PREC_MIXED;
+  bse_clear_counters<<<1, 16, 0, st>>>(cnt, 16);
+  LDGA_HIP_TRY(ctx, hipGetLastError());
   int rc;
-  if (precision == LDGA_PREC_FP64 && !use_register_gj()) {
+  if (precision == LDGA_PREC_FP64 && !register_gj) {
     rc = launch_fp64(ctx, N, Nq, Nw, Nc, fr->n_inu_shell, Gam, chi0, s, U, beta, chi, gamma, cnt, nullptr, nullptr, 0,
                      st);
   } else if (precision == LDGA_PREC_FP64) {

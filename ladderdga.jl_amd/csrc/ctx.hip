@@ -62,6 +62,11 @@ int ldga_create(int device, ldga_ctx **out) {
     delete ctx;
     return LDGA_ERR_HIP;
   }
+  if (ldga_noalloc_setup(ctx) != LDGA_OK) {
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return LDGA_ERR_HIP;
+  }
   *out = ctx;
   return LDGA_OK;
 }

@@ -75,7 +75,14 @@ struct ldga_ctx {
   OctTables oct{};
   // cached device buffers (grow-only, keyed by role)
   std::map<std::string, DevBuf> bufs;
+  // device address of this device's copy of the allocation-free entry
+  // point's static scratch array (set by ldga_create)
+  void *noalloc_scratch = nullptr;
 };
+
+// looks up the static scratch array of the allocation-free entry point on the
+// current device (hipGetSymbolAddress); called once per context at creation
+int ldga_noalloc_setup(ldga_ctx *ctx);
 
 // error plumbing ------------------------------------------------------------
 int ldga_fail(ldga_ctx *ctx, int code, const char *fmt, ...);
@@ -98,6 +105,9 @@ int launch_bubble(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double2
                   double2 *chi0, hipStream_t st);
 int launch_bse(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double U, double beta, const double2 *Gam,
                const double2 *chi0, double *chi, double2 *gamma, ldga_bse_report *rep, hipStream_t st);
+int launch_bse_with(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double U, double beta,
+                    const double2 *Gam, const double2 *chi0, double *chi, double2 *gamma, ldga_bse_report *rep,
+                    hipStream_t st, unsigned long long *cnt, int *redo, bool register_gj);
 int launch_lambda0(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double2 *F, const double2 *chi0,
                    double2 *lambda0, hipStream_t st);
 int launch_sum_kw(ldga_ctx *ctx, const double *chi, int Nw, int w_lo, int w_hi, double c2, double beta, double lam,

@@ -30,7 +30,7 @@ EXPORTED = [
     "ldga_version", "ldga_create", "ldga_destroy", "ldga_last_error", "ldga_sync",
     "ldga_set_kgrid", "ldga_default_wedge",
     "ldga_g_tables", "ldga_g_tables_dev", "ldga_bubble", "ldga_bubble_dev",
-    "ldga_chi_gamma", "ldga_chi_gamma_dev",
+    "ldga_chi_gamma", "ldga_chi_gamma_dev", "ldga_chi_gamma_noalloc_dev",
     "ldga_lambda0", "ldga_lambda0_dev", "ldga_F_from_chi", "ldga_F_from_chi_dev", "ldga_gen_chi", "ldga_gen_chi_dev",
     "ldga_sum_kw", "ldga_sum_kw_dev", "ldga_sum_w", "ldga_sum_w_dev",
     "ldga_sigma", "ldga_sigma_dev",
@@ -95,6 +95,8 @@ def lib():
             "ldga_chi_gamma": (i32, [vp, i32, i32, C.POINTER(Freq), dbl, dbl, vp, vp, vp, vp, C.POINTER(BseReport)]),
             "ldga_chi_gamma_dev": (i32, [vp, i32, i32, C.POINTER(Freq), dbl, dbl, vp, vp, vp, vp,
                                          C.POINTER(BseReport), vp]),
+            "ldga_chi_gamma_noalloc_dev": (i32, [vp, i32, i32, C.POINTER(Freq), dbl, dbl, vp, vp, vp, vp,
+                                                 C.POINTER(BseReport), vp]),
             "ldga_lambda0": (i32, [vp, C.POINTER(Freq), dbl, vp, vp, vp]),
             "ldga_F_from_chi": (i32, [vp, C.POINTER(Freq), dbl, i32, vp, vp, vp]),
             "ldga_gen_chi": (i32, [vp, C.POINTER(Freq), vp, vp, vp]),
