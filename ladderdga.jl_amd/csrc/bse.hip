@@ -309,7 +309,7 @@ __global__ void bse_clear_counters(unsigned long long *cnt, int n) {
 int launch_gen_chi(ldga_ctx *ctx, const ldga_freq *fr, const double2 *Gam, const double2 *chi0, double2 *inv,
                    hipStream_t st) {
   const int N = 2 * fr->n_inu, Nw = 2 * fr->n_iw + 1, Nq = ctx->Nq, Nc = N + 2 * fr->n_inu_shell;
-  unsigned long long *cnt = (unsigned long long *)ldga_buf(ctx, "gen_chi_counters", 8 * 8);
+  unsigned long long *cnt = (unsigned long long *)ldga_scratch(ctx, "gen_chi_counters", 8 * 8, st);
   if (!cnt) return LDGA_ERR_HIP;
   bse_clear_counters<<<1, 8, 0, st>>>(cnt, 8);
   LDGA_HIP_TRY(ctx, hipGetLastError());
@@ -337,12 +337,12 @@ int launch_bse(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double 
   // counters: [0] zero pivots, [1] refinement steps max, [2] unconverged refinements,
   // [3] largest final refinement ratio (bits), [5] refinement steps summed over the
   // matrices; [8..15]: the same for the FP64 re-solve
-  unsigned long long *cnt = (unsigned long long *)ldga_buf(ctx, "bse_counters", 16 * 8);
+  unsigned long long *cnt = (unsigned long long *)ldga_scratch(ctx, "bse_counters", 16 * 8, st);
   if (!cnt) return LDGA_ERR_HIP;
   // mixed mode: the refinement's failures are listed (room for every matrix)
   // and re-solved in FP64
   int *redo = nullptr;
-  if (precision == LDGA_PREC_MIXED && !(redo = (int *)ldga_buf(ctx, "bse_redo", (size_t)Nq * Nw * sizeof(int))))
+  if (precision == LDGA_PREC_MIXED && !(redo = (int *)ldga_scratch(ctx, "bse_redo", (size_t)Nq * Nw * sizeof(int), st)))
     return LDGA_ERR_HIP;
   return launch_bse_with(ctx, s, precision, fr, U, beta, Gam, chi0, chi, gamma, rep, st, cnt, redo, use_register_gj());
 }

@@ -191,7 +191,7 @@ int launch_bse_gmem_f64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell,
   const size_t slot = (size_t)N * N * sizeof(double2);
   int grid = n_items < 1024 ? n_items : 1024;
   while (grid > 1 && (size_t)grid * slot > (size_t(1) << 30)) grid /= 2;
-  double2 *ws = (double2 *)ldga_buf(ctx, "bse_gmem_ws", (size_t)grid * slot);
+  double2 *ws = (double2 *)ldga_scratch(ctx, "bse_gmem_ws", (size_t)grid * slot, st);
   if (!ws) return LDGA_ERR_HIP;
   const size_t lds = (size_t)N * (3 * sizeof(double2) + 2 * sizeof(int));
   bse_gmem_f64_kernel<<<dim3((unsigned)grid), dim3(GM_THREADS), lds, st>>>(N, Nq, Nc, shell, Gam, chi0, s, U, beta,

@@ -40,6 +40,55 @@ void *ldga_buf(ldga_ctx *ctx, const char *role, size_t bytes) {
 
 hipStream_t ldga_stream(ldga_ctx *ctx, void *stream) { return stream ? (hipStream_t)stream : ctx->stream; }
 
+// Scratch of the launchers, one buffer per (role, stream): _dev calls on
+// different caller streams never share partial sums, counters or lists.
+void *ldga_scratch(ldga_ctx *ctx, const char *role, size_t bytes, hipStream_t st) {
+  char key[96];
+  snprintf(key, sizeof key, "%s#%p", role, (void *)st);
+  DevBuf &b = ctx->bufs[key];
+  if (b.bytes >= bytes && b.ptr) return b.ptr;
+  if (b.ptr) {
+    hipStreamSynchronize(st);
+    hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  if (hipMalloc(&b.ptr, std::max<size_t>(bytes, 256)) != hipSuccess) {
+    b.ptr = nullptr;
+    ldga_fail(ctx, LDGA_ERR_HIP, "hipMalloc of %zu bytes for %s failed", bytes, role);
+    return nullptr;
+  }
+  b.bytes = bytes;
+  b.fresh = true;
+  return b.ptr;
+}
+
+namespace {
+__global__ void zero_u32_kernel(unsigned *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+}  // namespace
+
+// n ticket counters of device_sum (ldga_reduce.h): zeroed by a kernel when the
+// buffer is new; the kernels that use them leave them at zero
+unsigned *ldga_tickets(ldga_ctx *ctx, const char *role, int n, hipStream_t st) {
+  unsigned *p = (unsigned *)ldga_scratch(ctx, role, (size_t)n * sizeof(unsigned), st);
+  if (!p) return nullptr;
+  char key[96];
+  snprintf(key, sizeof key, "%s#%p", role, (void *)st);
+  DevBuf &b = ctx->bufs[key];
+  if (b.fresh) {
+    zero_u32_kernel<<<(n + 255) / 256, 256, 0, st>>>(p, n);
+    if (hipGetLastError() != hipSuccess) {
+      ldga_fail(ctx, LDGA_ERR_HIP, "clearing the %s tickets failed", role);
+      return nullptr;
+    }
+    b.fresh = false;
+  }
+  return p;
+}
+
 int ldga_check_freq(ldga_ctx *ctx, const ldga_freq *fr) {
   if (!fr) return ldga_fail(ctx, LDGA_ERR_ARG, "freq is NULL");
   if (fr->n_iw < 0 || fr->n_inu < 1 || fr->n_inu_shell < 0 || fr->n_fft < 1 || (fr->shift != 0 && fr->shift != 1))
@@ -72,8 +121,8 @@ int ldga_create(int device, ldga_ctx **out) {
 }
 
 static void free_kgrid(ldga_ctx *ctx) {
-  const void *p[] = {ctx->oct.oct2wedge, ctx->oct.wedge2oct, ctx->oct.octfull, ctx->oct.octw,
-                     ctx->oct.octphase, ctx->oct.cosT,      ctx->oct.kmult,   ctx->oct.eps};
+  const void *p[] = {ctx->oct.oct2wedge, ctx->oct.wedge2oct, ctx->oct.octfull, ctx->oct.octw,  ctx->oct.octphase,
+                     ctx->oct.octpw,     ctx->oct.cosT,      ctx->oct.kmult,   ctx->oct.eps};
   for (const void *q : p)
     if (q) hipFree(const_cast<void *>(q));
   ctx->oct = OctTables{};
@@ -204,7 +253,7 @@ int ldga_set_kgrid(ldga_ctx *ctx, int D, int Ns, int Nq, const int32_t *wedge, c
   }
   std::vector<int32_t> o2w(NO), w2o(Nq), ofull(NO);
   std::vector<double> ow(NO), cosT(Ns), kmult(Nq, 0.0);
-  std::vector<double2> oph(NO);
+  std::vector<double2> oph(NO), opw(NO);
   const int Nk = D == 2 ? Ns * Ns : Ns * Ns * Ns;
   for (int o = 0; o < NO; ++o) {
     int c[3] = {o % n, (o / n) % n, D == 3 ? o / (n * n) : 0};
@@ -219,6 +268,7 @@ int ldga_set_kgrid(ldga_ctx *ctx, int D, int Ns, int Nq, const int32_t *wedge, c
     ow[o] = w;
     const double ang = -2.0 * M_PI * (double)(sum % Ns) / (double)Ns;
     oph[o] = make_double2(std::cos(ang), std::sin(ang));
+    opw[o] = make_double2(w * oph[o].x, w * oph[o].y);
   }
   for (int q = 0; q < Nq; ++q) {
     const int *c = &wedge[D * q];
@@ -256,6 +306,8 @@ int ldga_set_kgrid(ldga_ctx *ctx, int D, int Ns, int Nq, const int32_t *wedge, c
   T.octw = (double *)p;
   if ((rc = up(oph.data(), NO * 16, &p))) return rc;
   T.octphase = (double2 *)p;
+  if ((rc = up(opw.data(), NO * 16, &p))) return rc;
+  T.octpw = (double2 *)p;
   if ((rc = up(cosT.data(), Ns * 8, &p))) return rc;
   T.cosT = (double *)p;
   if ((rc = up(kmult.data(), Nq * 8, &p))) return rc;

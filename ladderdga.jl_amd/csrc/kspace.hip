@@ -9,75 +9,120 @@
 // conv[q] = 1/Nk^2 sum_R A(R) B(-R) exp(-i q.R).  All functions of the path
 // are even in every axis, so the sum folds onto the octant R in [0,Ns/2]^D
 // with multiplicity w(R) and a separable cosine kernel: one small dense
-// "mode product" per axis, evaluated in LDS.  The last axis is evaluated only
-// at the wedge points that are stored.
+// "mode product" per axis, evaluated in LDS; the wedge points are then read
+// from the transformed octant.
 #include <algorithm>
 
 #include "ldga_internal.h"
+#include "ldga_reduce.h"
 
 namespace {
 
 constexpr int kNT = 256;  // threads per octant workgroup
 
 // In-place mode product along axis `ax` over the whole octant held in LDS:
-//   buf[o] <- sum_r cos(2 pi q_ax r / Ns) * buf[o with coordinate ax = r]
-template <int MAXPT>
+//   buf[o] <- sum_r cos(2 pi q_ax r / Ns) * buf[o with coordinate ax = r].
+// The octant splits into lines along `ax` (the points sharing the other
+// coordinates).  Outputs q and Ns/2 - q share their products, since
+// cos(2 pi (Ns/2 - q) r / Ns) = (-1)^r cos(2 pi q r / Ns): with E_q and O_q the
+// sums over even and odd r, y[q] = E_q + O_q and y[Ns/2 - q] = E_q - O_q, for
+// q = 0 .. Ns/4.  Thread (line, q-block) reads the n values of its line once
+// and forms kQ such pairs; the cosines come from the Chebyshev recurrence
+// c(r+1) = 2 cos(theta) c(r) - c(r-1), seeded with the exact cos(theta) = cs[q]
+// (error below 2e-14 for Ns <= 64).  Lines go in rounds of kNT / nqb lines; a
+// round writes only its own lines, so one barrier per round separates its
+// reads from its writes.
+constexpr int kQ = 4;
 __device__ __forceinline__ void oct_pass(double2 *buf, const double *cs, int ax, const OctTables &T) {
-  const int stride = T.stride[ax];
-  const int n = T.n, Ns = T.Ns;
-  double2 acc[MAXPT];
+  const int n = T.n, half = T.Ns / 2, stride = T.stride[ax];
+  const int np = T.Ns / 4 + 1;  // q = 0 .. Ns/4 with their partners Ns/2 - q cover 0 .. Ns/2
+  const int nqb = (np + kQ - 1) / kQ, nlines = T.NO / n, per_round = kNT / nqb;
+  const int li = threadIdx.x / nqb, q0 = (threadIdx.x - li * nqb) * kQ;
+  double tw[kQ];
 #pragma unroll
-  for (int p = 0; p < MAXPT; ++p) {
-    const int o = threadIdx.x + p * kNT;
-    double2 s = cz(0.0, 0.0);
-    if (o < T.NO) {
-      const int qd = (o / stride) % n;
-      const int base = o - qd * stride;
-      int m = 0;
-      for (int r = 0; r < n; ++r) {
-        const double c = cs[m];
-        const double2 v = buf[base + r * stride];
-        s.x = fma(c, v.x, s.x);
-        s.y = fma(c, v.y, s.y);
-        m += qd;
-        m -= (m >= Ns) ? Ns : 0;
+  for (int j = 0; j < kQ; ++j) tw[j] = (q0 + j < np) ? 2.0 * cs[q0 + j] : 0.0;
+  for (int l0 = 0; l0 < nlines; l0 += per_round) {
+    const int line = l0 + li;
+    const bool active = li < per_round && line < nlines;
+    // octant index of the line's point with coordinate 0 on axis ax
+    const int base = (line % stride) + (line / stride) * stride * n;
+    double2 E[kQ], O[kQ];
+#pragma unroll
+    for (int j = 0; j < kQ; ++j) E[j] = O[j] = cz(0.0, 0.0);
+    if (active) {
+      double c[kQ], cm[kQ];
+#pragma unroll
+      for (int j = 0; j < kQ; ++j) {
+        c[j] = 1.0;            // cos(0)
+        cm[j] = 0.5 * tw[j];   // cos(-theta)
+      }
+      for (int r = 0; r < n; r += 2) {
+        const double2 xe = buf[base + r * stride];
+        const double2 xo = (r + 1 < n) ? buf[base + (r + 1) * stride] : cz(0.0, 0.0);
+#pragma unroll
+        for (int j = 0; j < kQ; ++j) {
+          E[j].x = fma(c[j], xe.x, E[j].x);
+          E[j].y = fma(c[j], xe.y, E[j].y);
+          const double c1 = fma(tw[j], c[j], -cm[j]);
+          O[j].x = fma(c1, xo.x, O[j].x);
+          O[j].y = fma(c1, xo.y, O[j].y);
+          cm[j] = c1;
+          c[j] = fma(tw[j], c1, -c[j]);
+        }
       }
     }
-    acc[p] = s;
-  }
-  __syncthreads();
+    __syncthreads();
+    if (active) {
 #pragma unroll
-  for (int p = 0; p < MAXPT; ++p) {
-    const int o = threadIdx.x + p * kNT;
-    if (o < T.NO) buf[o] = acc[p];
+      for (int j = 0; j < kQ; ++j) {
+        const int q = q0 + j;
+        if (q < np) {
+          buf[base + q * stride] = cadd(E[j], O[j]);
+          if (half - q != q) buf[base + (half - q) * stride] = csub(E[j], O[j]);
+        }
+      }
+    }
   }
   __syncthreads();
-}
-
-// The last axis at one wedge point: sum_r cos(2 pi q_last r/Ns) * buf[...]
-__device__ __forceinline__ double2 oct_last_at(const double2 *buf, const double *cs, int q, const OctTables &T) {
-  const int ax = T.D - 1;
-  const int stride = T.stride[ax];
-  const int o = T.wedge2oct[q];
-  const int qd = (o / stride) % T.n;
-  const int base = o - qd * stride;
-  double2 s = cz(0.0, 0.0);
-  int m = 0;
-  for (int r = 0; r < T.n; ++r) {
-    const double c = cs[m];
-    const double2 v = buf[base + r * stride];
-    s.x = fma(c, v.x, s.x);
-    s.y = fma(c, v.y, s.y);
-    m += qd;
-    m -= (m >= T.Ns) ? T.Ns : 0;
-  }
-  return s;
 }
 
 __device__ __forceinline__ int trunc_half(int x) { return x / 2; }  // C division truncates toward 0 like Julia trunc
 
+// Wedge points per thread (q = threadIdx.x + k kNT): Nq <= 969 for the grids
+// ldga_set_kgrid accepts (2D Ns <= 64: 561, 3D Ns <= 32: 969).
+constexpr int kMaxQ = 4;
+
+// This thread's octant points o = threadIdx.x + p kNT, p < MAXPT: their index
+// in the full grid and octphase * octw, loaded once with every load in flight
+// together (the points past NO get index 0 and weight 0).
+template <int MAXPT>
+struct OctPts {
+  int f[MAXPT];
+  double2 pw[MAXPT];
+  __device__ __forceinline__ void load(const OctTables &T) {
+#pragma unroll
+    for (int p = 0; p < MAXPT; ++p) {
+      const int o = threadIdx.x + p * kNT;
+      const bool v = o < T.NO;
+      f[p] = v ? T.octfull[o] : 0;
+      pw[p] = v ? T.octpw[o] : cz(0.0, 0.0);
+    }
+  }
+};
+
+// the octant index of this thread's wedge points (index 0 past Nq), for the
+// final reads of the transformed octant
+__device__ __forceinline__ void load_w2o(const OctTables &T, int (&w2o)[kMaxQ]) {
+#pragma unroll
+  for (int k = 0; k < kMaxQ; ++k) {
+    const int q = threadIdx.x + k * kNT;
+    w2o[k] = q < T.Nq ? T.wedge2oct[q] : 0;
+  }
+}
+
 // ---------------------------------------------------------------------------
-// bubble: one workgroup per (nu, w) pair
+// bubble: one workgroup per (nu, w) pair (the octant transform on all D axes, then
+// the wedge points read from it)
 // ---------------------------------------------------------------------------
 template <int MAXPT>
 __global__ __launch_bounds__(kNT) void bubble_kernel(OctTables T, int Nnu, int n_inu_tot, int n_iw, int shift,
@@ -95,15 +140,36 @@ __global__ __launch_bounds__(kNT) void bubble_kernel(OctTables T, int Nnu, int n
   const double2 *a = Gfft + (size_t)(nun - fft_first) * fs;
   const double2 *b = Grfft + (size_t)(nun + wn - fft_first) * fs;
   for (int m = threadIdx.x; m < T.Ns; m += kNT) cs[m] = T.cosT[m];
-  for (int o = threadIdx.x; o < T.NO; o += kNT) {
-    const int f = T.octfull[o];
-    const double2 p = cmul(cmul(a[f], b[f]), T.octphase[o]);
-    buf[o] = cscale(p, T.octw[o]);
+  int w2o[kMaxQ];
+  load_w2o(T, w2o);  // used after the passes: their latency hides the loads
+  OctPts<MAXPT> P;
+  P.load(T);
+  // the products in batches of up to 10 points, every load of a batch in flight together
+  constexpr int B = MAXPT < 10 ? MAXPT : 10;
+#pragma unroll
+  for (int p0 = 0; p0 < MAXPT; p0 += B) {
+    double2 va[B], vb[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      if (p0 + k < MAXPT) {
+        va[k] = a[P.f[p0 + k]];
+        vb[k] = b[P.f[p0 + k]];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const int o = threadIdx.x + (p0 + k) * kNT;
+      if (p0 + k < MAXPT && o < T.NO) buf[o] = cmul(cmul(va[k], vb[k]), P.pw[p0 + k]);
+    }
   }
   __syncthreads();
-  for (int ax = 0; ax < T.D - 1; ++ax) oct_pass<MAXPT>(buf, cs, ax, T);
+  for (int ax = 0; ax < T.D; ++ax) oct_pass(buf, cs, ax, T);
   double2 *out = chi0 + (size_t)T.Nq * (nui + (size_t)Nnu * wi);
-  for (int q = threadIdx.x; q < T.Nq; q += kNT) out[q] = cscale(oct_last_at(buf, cs, q, T), scale);
+#pragma unroll
+  for (int k = 0; k < kMaxQ; ++k) {
+    const int q = threadIdx.x + k * kNT;
+    if (q < T.Nq) out[q] = cscale(buf[w2o[k]], scale);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -144,13 +210,26 @@ __global__ __launch_bounds__(kNT) void eom_accum_kernel(OctTables T, int Nw, int
                                                         double2 *__restrict__ partial) {
   extern __shared__ double2 smem[];
   double2 *buf = smem;
-  double *cs = reinterpret_cast<double *>(smem + T.NO);
+  double2 *Kw = smem + T.NO;  // K on the wedge points
+  double *cs = reinterpret_cast<double *>(Kw + T.Nq);
   const int nuii = blockIdx.x % numax;
   const int chunk = blockIdx.x / numax;
   for (int m = threadIdx.x; m < T.Ns; m += kNT) cs[m] = T.cosT[m];
   double2 S[MAXPT];
 #pragma unroll
   for (int p = 0; p < MAXPT; ++p) S[p] = cz(0.0, 0.0);
+  // this thread's octant points: full-grid index, phase x weight, wedge index and
+  // weight, loaded once for the whole omega chunk
+  OctPts<MAXPT> P;
+  P.load(T);
+  int o2w[MAXPT];
+  double ow[MAXPT];
+#pragma unroll
+  for (int p = 0; p < MAXPT; ++p) {
+    const int o = threadIdx.x + p * kNT;
+    o2w[p] = o < T.NO ? T.oct2wedge[o] : 0;
+    ow[p] = o < T.NO ? T.octw[o] : 0.0;
+  }
   const int w0 = chunk * wchunk, w1 = min(Nw, w0 + wchunk);
   const size_t fs = (size_t)T.Nk;
   for (int wi = w0; wi < w1; ++wi) {
@@ -158,26 +237,32 @@ __global__ __launch_bounds__(kNT) void eom_accum_kernel(OctTables T, int Nw, int
     // nu-window of gamma at this w: nu0 index (1-based) = n_inu + shift*trunc(wn/2) + 1  (helpers.jl:58)
     const int nui = n_inu + shift * trunc_half(wn) + nuii;  // 0-based index of nu_n = nuii
     if (nui > 2 * n_inu - 1) continue;                     // nu list ends at 2 n_inu (singleCore.jl:59-60)
-    __syncthreads();                                        // previous iteration finished with buf
-    for (int o = threadIdx.x; o < T.NO; o += kNT) {
-      const int q = T.oct2wedge[o];
+    // K on the wedge points, read once and coalesced (q is the fastest index of
+    // gamma, lambda0 and chi), then expanded to the weighted octant from LDS
+    for (int q = threadIdx.x; q < T.Nq; q += kNT) {
       double cm = chim[q + (size_t)T.Nq * wi], cd = chid[q + (size_t)T.Nq * wi];
       if (lam_m != 0.0) cm = cm / (lam_m * cm + 1.0);
       if (lam_d != 0.0) cd = cd / (lam_d * cd + 1.0);
       const size_t idx = q + (size_t)T.Nq * (nui + (size_t)Nnu * wi);
-      const double2 K = eom_value(kind, U, gm[idx], gd[idx], cm, cd, l0[idx]);
-      buf[o] = cscale(K, T.octw[o]);
+      Kw[q] = eom_value(kind, U, gm[idx], gd[idx], cm, cd, l0[idx]);
     }
-    __syncthreads();
-    for (int ax = 0; ax < T.D; ++ax) oct_pass<MAXPT>(buf, cs, ax, T);
-    const double2 *g = Grfft + (size_t)(nuii + wn - fft_first) * fs;
+    __syncthreads();  // Kw written; the previous iteration is done with buf
 #pragma unroll
     for (int p = 0; p < MAXPT; ++p) {
       const int o = threadIdx.x + p * kNT;
-      if (o < T.NO) {
-        const double2 gb = cmul(g[T.octfull[o]], T.octphase[o]);
-        S[p] = cadd(S[p], cscale(cmul(buf[o], gb), T.octw[o]));
-      }
+      if (o < T.NO) buf[o] = cscale(Kw[o2w[p]], ow[p]);
+    }
+    __syncthreads();
+    // G(nu + w) at this thread's points, loaded before the passes, which hide the latency
+    const double2 *g = Grfft + (size_t)(nuii + wn - fft_first) * fs;
+    double2 gv[MAXPT];
+#pragma unroll
+    for (int p = 0; p < MAXPT; ++p) gv[p] = g[P.f[p]];
+    for (int ax = 0; ax < T.D; ++ax) oct_pass(buf, cs, ax, T);
+#pragma unroll
+    for (int p = 0; p < MAXPT; ++p) {
+      const int o = threadIdx.x + p * kNT;
+      if (o < T.NO) S[p] = cadd(S[p], cmul(buf[o], cmul(gv[p], P.pw[p])));
     }
   }
   double2 *dst = partial + (size_t)T.NO * (nuii + (size_t)numax * chunk);
@@ -199,21 +284,36 @@ __global__ __launch_bounds__(kNT) void eom_final_kernel(OctTables T, int numax, 
   double *cs = reinterpret_cast<double *>(smem + T.NO);
   const int nuii = blockIdx.x;
   for (int m = threadIdx.x; m < T.Ns; m += kNT) cs[m] = T.cosT[m];
+  int w2o[kMaxQ];
+  load_w2o(T, w2o);
+  // the chunk partials of this nu, four chunks in flight per step
+  const size_t cstride = (size_t)T.NO * numax;
+  const double2 *pp = partial + (size_t)T.NO * nuii;
   for (int o = threadIdx.x; o < T.NO; o += kNT) {
     double2 s = cz(0.0, 0.0);
-    for (int c = 0; c < nchunks; ++c) s = cadd(s, partial[(size_t)T.NO * (nuii + (size_t)numax * c) + o]);
+    int c = 0;
+    for (; c + 4 <= nchunks; c += 4) {
+      const double2 a0 = pp[cstride * c + o], a1 = pp[cstride * (c + 1) + o];
+      const double2 a2 = pp[cstride * (c + 2) + o], a3 = pp[cstride * (c + 3) + o];
+      s = cadd(s, cadd(cadd(a0, a1), cadd(a2, a3)));
+    }
+    for (; c < nchunks; ++c) s = cadd(s, pp[cstride * c + o]);
     buf[o] = s;
   }
   __syncthreads();
-  for (int ax = 0; ax < T.D - 1; ++ax) oct_pass<MAXPT>(buf, cs, ax, T);
+  for (int ax = 0; ax < T.D; ++ax) oct_pass(buf, cs, ax, T);
   // tail/(i nu_n) = -i * tail * beta / ((2n+1) pi)   (singleCore.jl:78-79)
   double tail = 0.0;
   if (add_tail) tail = -U * (tail_sum[0] - chiloc_m_sum);
   const double tail_im = -tail * beta / ((2.0 * nuii + 1.0) * M_PI);
   double2 *out = sigma + (size_t)T.Nq * nuii;
-  for (int q = threadIdx.x; q < T.Nq; q += kNT) {
-    const double2 v = cscale(oct_last_at(buf, cs, q, T), scale);
-    out[q] = cz(v.x + hartree, v.y + tail_im);
+#pragma unroll
+  for (int k = 0; k < kMaxQ; ++k) {
+    const int q = threadIdx.x + k * kNT;
+    if (q < T.Nq) {
+      const double2 v = cscale(buf[w2o[k]], scale);
+      out[q] = cz(v.x + hartree, v.y + tail_im);
+    }
   }
 }
 
@@ -242,50 +342,36 @@ __global__ void lambda0_kernel(int Nq, int Nnu, int Nw, int shell, double inv_b2
 }
 
 // ---------------------------------------------------------------------------
-// sum_kw: one workgroup, FP64 tree reduction
+// sum_kw: a device-wide FP64 reduction over the usable (q, w) block; the last
+// workgroup adds the tail terms and writes the three sums
 // ---------------------------------------------------------------------------
-constexpr int kSumNT = 1024;
-__global__ __launch_bounds__(kSumNT) void sum_kw_kernel(const double *__restrict__ chi, const double *__restrict__ kmult,
+constexpr int kSumNT = 1024;  // the single-workgroup k integral below
+constexpr int kSumRB = 256, kSumGrid = 64;
+__global__ __launch_bounds__(kSumRB) void sum_kw_kernel(const double *__restrict__ chi, const double *__restrict__ kmult,
                                                         int Nq, int w_lo, int w_hi, double Nk, double lam, double c2,
-                                                        double beta, int n_iw, double *__restrict__ out) {
-  __shared__ double red[2][kSumNT / 64];
-  double s0 = 0.0, s1 = 0.0;
-  const int nw = w_hi - w_lo + 1;
-  const size_t total = (size_t)Nq * nw;
-  for (size_t i = threadIdx.x; i < total; i += kSumNT) {
-    const int q = (int)(i % Nq);
-    const int w = w_lo + (int)(i / Nq);
-    const double x = chi[q + (size_t)Nq * w];
+                                                        double beta, int n_iw, double *part, unsigned *ticket,
+                                                        double *__restrict__ out) {
+  double v[2] = {0.0, 0.0};
+  const int nw = w_hi - w_lo + 1, total = Nq * nw, stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    const int q = i % Nq;
+    const double x = chi[i + Nq * w_lo];  // chi[q + Nq w], w = w_lo + i / Nq
     const double f = (lam != 0.0) ? x / (lam * x + 1.0) : x;
-    s0 = fma(kmult[q], f, s0);
-    s1 = fma(kmult[q], -f * f, s1);
+    v[0] = fma(kmult[q], f, v[0]);
+    v[1] = fma(kmult[q], -f * f, v[1]);
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    s0 += __shfl_xor(s0, off);
-    s1 += __shfl_xor(s1, off);
+  double tot[2];
+  if (!ldga_red::device_sum<2>(v, part, ticket, tot) || threadIdx.x != 0) return;
+  // Re(c2/(i w_n)^2) = -c2 beta^2 / (4 pi^2 m^2), zero at m = 0  (DataTypes.jl:223-225)
+  double tail = 0.0;
+  for (int w = w_lo; w <= w_hi; ++w) {
+    const int m = w - n_iw;
+    if (m != 0) tail += -c2 * beta * beta / (4.0 * M_PI * M_PI * (double)m * (double)m);
   }
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (lane == 0) {
-    red[0][wv] = s0;
-    red[1][wv] = s1;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double a = 0.0, b = 0.0, tail = 0.0;
-    for (int i = 0; i < kSumNT / 64; ++i) {
-      a += red[0][i];
-      b += red[1][i];
-    }
-    // Re(c2/(i w_n)^2) = -c2 beta^2 / (4 pi^2 m^2), zero at m = 0  (DataTypes.jl:223-225)
-    for (int w = w_lo; w <= w_hi; ++w) {
-      const int m = w - n_iw;
-      if (m != 0) tail += -c2 * beta * beta / (4.0 * M_PI * M_PI * (double)m * (double)m);
-    }
-    const double konst = -beta * beta / 12.0 * c2;
-    out[0] = (a / Nk - tail + konst) / beta;
-    out[1] = (b / Nk - tail + konst) / beta;
-    out[2] = (b / Nk) / beta;
-  }
+  const double konst = -beta * beta / 12.0 * c2;
+  out[0] = (tot[0] / Nk - tail + konst) / beta;
+  out[1] = (tot[1] / Nk - tail + konst) / beta;
+  out[2] = (tot[1] / Nk) / beta;
 }
 
 // ---------------------------------------------------------------------------
@@ -357,7 +443,7 @@ __global__ __launch_bounds__(kNT) void eom_adjoint_kernel(OctTables T, int Nw, i
   for (int o = threadIdx.x; o < T.NO; o += kNT)
     buf[o] = cscale(W[T.oct2wedge[o] + (size_t)T.Nq * nuii], T.octw[o]);
   __syncthreads();
-  for (int ax = 0; ax < T.D; ++ax) oct_pass<MAXPT>(buf, cs, ax, T);
+  for (int ax = 0; ax < T.D; ++ax) oct_pass(buf, cs, ax, T);
   double2 What[MAXPT];
 #pragma unroll
   for (int p = 0; p < MAXPT; ++p) {
@@ -379,9 +465,9 @@ __global__ __launch_bounds__(kNT) void eom_adjoint_kernel(OctTables T, int Nw, i
       if (o < T.NO) buf[o] = cscale(cmul(What[p], cmul(g[T.octfull[o]], T.octphase[o])), T.octw[o]);
     }
     __syncthreads();
-    for (int ax = 0; ax < T.D - 1; ++ax) oct_pass<MAXPT>(buf, cs, ax, T);
+    for (int ax = 0; ax < T.D; ++ax) oct_pass(buf, cs, ax, T);
     for (int q = threadIdx.x; q < T.Nq; q += kNT) {
-      const double2 t = cscale(oct_last_at(buf, cs, q, T), scale);
+      const double2 t = cscale(buf[T.wedge2oct[q]], scale);
       double cm = chim[q + (size_t)T.Nq * wi], cd = chid[q + (size_t)T.Nq * wi];
       if (lam_m != 0.0) cm = cm / (lam_m * cm + 1.0);
       if (lam_d != 0.0) cd = cd / (lam_d * cd + 1.0);
@@ -461,7 +547,7 @@ __global__ __launch_bounds__(kNT) void gtables_kernel(OctTables T, const double2
   for (int m = threadIdx.x; m < T.Ns; m += kNT) cs[m] = T.cosT[m];
   for (int o = threadIdx.x; o < T.NO; o += kNT) buf[o] = cscale(g[T.oct2wedge[o]], T.octw[o]);
   __syncthreads();
-  for (int ax = 0; ax < T.D; ++ax) oct_pass<MAXPT>(buf, cs, ax, T);
+  for (int ax = 0; ax < T.D; ++ax) oct_pass(buf, cs, ax, T);
   // the full grid, first axis fastest (column-major like the reference's arrays)
   double2 *a = Gfft + (size_t)T.Nk * f, *b = Grfft + (size_t)T.Nk * f;
   const int Nk = (int)T.Nk;
@@ -544,7 +630,7 @@ int launch_gtables(ldga_ctx *ctx, const ldga_freq *fr, const double2 *Gw, double
 int launch_lambda0(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double2 *F, const double2 *chi0,
                    double2 *lambda0, hipStream_t st) {
   const int Nnu = 2 * fr->n_inu, Nw = 2 * fr->n_iw + 1, Nq = ctx->Nq;
-  double2 *fsum = (double2 *)ldga_buf(ctx, "fsum", (size_t)Nnu * Nw * 16);
+  double2 *fsum = (double2 *)ldga_scratch(ctx, "fsum", (size_t)Nnu * Nw * 16, st);
   if (!fsum) return LDGA_ERR_HIP;
   fsum_kernel<<<(Nnu * Nw + 255) / 256, 256, 0, st>>>(Nnu, Nw, F, fsum);
   const size_t total = (size_t)Nq * Nnu * Nw;
@@ -599,7 +685,13 @@ int launch_sum_kw(ldga_ctx *ctx, const double *chi, int Nw, int w_lo, int w_hi, 
   if (Nw < 1 || (Nw & 1) == 0 || w_lo < 0 || w_hi >= Nw || w_lo > w_hi)
     return ldga_fail(ctx, LDGA_ERR_SHAPE, "bad omega range [%d,%d] of %d", w_lo, w_hi, Nw);
   const OctTables &T = ctx->oct;
-  sum_kw_kernel<<<1, kSumNT, 0, st>>>(chi, T.kmult, T.Nq, w_lo, w_hi, T.Nk, lam, c2, beta, (Nw - 1) / 2, out_dev);
+  double *part = (double *)ldga_scratch(ctx, "sumkw_part", 2 * kSumGrid * 8, st);
+  unsigned *ticket = ldga_tickets(ctx, "sumkw_ticket", 1, st);
+  if (!part || !ticket) return LDGA_ERR_HIP;
+  const int total = T.Nq * (w_hi - w_lo + 1);
+  const unsigned grid = (unsigned)std::min(kSumGrid, (total + kSumRB - 1) / kSumRB);
+  sum_kw_kernel<<<grid, kSumRB, 0, st>>>(chi, T.kmult, T.Nq, w_lo, w_hi, T.Nk, lam, c2, beta, (Nw - 1) / 2, part,
+                                         ticket, out_dev);
   LDGA_HIP_TRY(ctx, hipGetLastError());
   return LDGA_OK;
 }
@@ -620,9 +712,24 @@ int launch_sum_w(ldga_ctx *ctx, const double *chi, int Nq, int Nw, int w_lo, int
   return LDGA_OK;
 }
 
-// omega chunking of the EoM: enough (nu, chunk) workgroups to fill the device
-int eom_chunks(int Nw, int numax, int *wchunk) {
-  int nchunks = std::max(1, std::min(Nw, (2048 + numax - 1) / numax));
+// omega chunking of the EoM: as many (nu, chunk) workgroups as the device holds
+// at once (CUs x resident eom_accum workgroups per CU), so that they run in one
+// round; each chunk covers wchunk consecutive omegas.
+int eom_chunks(ldga_ctx *ctx, int Nw, int numax, int *wchunk) {
+  const OctTables &T = ctx->oct;
+  const int mp = oct_maxpt(T.NO);
+  const void *kern = mp <= 5 ? (const void *)eom_accum_kernel<5> : mp <= 9 ? (const void *)eom_accum_kernel<9>
+                                                                             : (const void *)eom_accum_kernel<20>;
+  const size_t smem = (size_t)T.NO * 16 + (size_t)T.Ns * 8 + (size_t)T.Nq * 16;
+  if (ctx->eom_resident_key != (size_t)T.NO * 4096 + T.Nq) {
+    int ncu = 0, per_cu = 0;
+    if (smem > 65536) hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kNT, smem);
+    ctx->eom_resident = std::max(1, ncu) * std::max(1, per_cu);
+    ctx->eom_resident_key = (size_t)T.NO * 4096 + T.Nq;
+  }
+  int nchunks = std::max(1, std::min(Nw, ctx->eom_resident / std::max(numax, 1)));
   *wchunk = (Nw + nchunks - 1) / nchunks;
   return (Nw + *wchunk - 1) / *wchunk;
 }
@@ -642,9 +749,9 @@ int launch_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int n
   if (fr->n_inu - fr->shift * (fr->n_iw / 2) < 0)
     return ldga_fail(ctx, LDGA_ERR_SHAPE, "nu window does not contain nu_n = 0 at every w");
   int wchunk = 1;
-  const int nchunks = eom_chunks(Nw, numax, &wchunk);
-  double2 *partial = (double2 *)ldga_buf(ctx, "eom_partial", (size_t)T.NO * numax * nchunks * 16);
-  double *tsum = (double *)ldga_buf(ctx, "eom_tail", 3 * 8);
+  const int nchunks = eom_chunks(ctx, Nw, numax, &wchunk);
+  double2 *partial = (double2 *)ldga_scratch(ctx, "eom_partial", (size_t)T.NO * numax * nchunks * 16, st);
+  double *tsum = (double *)ldga_scratch(ctx, "eom_tail", 3 * 8, st);
   if (!partial || !tsum) return LDGA_ERR_HIP;
   const bool add_tail = tc && eom_kind == LDGA_EOM_FULL;
   if (add_tail) {
@@ -652,7 +759,8 @@ int launch_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int n
     if (rc) return rc;
   }
   const size_t smem = (size_t)T.NO * 16 + (size_t)T.Ns * 8;
-  OCT_DISPATCH(T.NO, smem, eom_accum_kernel, numax * nchunks, st, T, Nw, Nnu, fr->n_inu, fr->n_iw, fr->shift,
+  const size_t smem_acc = smem + (size_t)T.Nq * 16;
+  OCT_DISPATCH(T.NO, smem_acc, eom_accum_kernel, numax * nchunks, st, T, Nw, Nnu, fr->n_inu, fr->n_iw, fr->shift,
                fr->fft_first, numax, wchunk, eom_kind, mp->U, lam_m, lam_d, chi_m, chi_d, gamma_m, gamma_d, lambda0,
                Grfft, partial);
   const double hartree = (eom_kind == LDGA_EOM_FULL || eom_kind == LDGA_EOM_REST) ? mp->n * mp->U / 2.0 : 0.0;
@@ -670,8 +778,8 @@ int launch_eom_adjoint(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp,
   const OctTables &T = ctx->oct;
   const int Nnu = 2 * fr->n_inu, Nw = 2 * fr->n_iw + 1;
   int wchunk = 1;
-  const int nchunks = eom_chunks(Nw, numax, &wchunk);
-  double2 *partial = (double2 *)ldga_buf(ctx, "adj_partial", (size_t)2 * numax * nchunks * 16);
+  const int nchunks = eom_chunks(ctx, Nw, numax, &wchunk);
+  double2 *partial = (double2 *)ldga_scratch(ctx, "adj_partial", (size_t)2 * numax * nchunks * 16, st);
   if (!partial) return LDGA_ERR_HIP;
   const size_t smem = (size_t)T.NO * 16 + (size_t)T.Ns * 8;
   const double scale = 1.0 / (T.Nk * T.Nk);

@@ -2,35 +2,36 @@
 //   EoM Σ(λm, λd)  ->  μ refit (device secant)  ->  calc_E, filling_pos, Σχ sums
 // and, on request, its Jacobian by implicit differentiation plus the EoM
 // adjoint (launch_eom_adjoint).
+//
+// Every sum runs across the device: a reduction kernel leaves one partial per
+// workgroup, and the last workgroup to finish (an atomic ticket, no waiting)
+// adds the partials in a fixed order and finalises.  The μ secant is a short
+// sequence of such launches, one filling evaluation each, with its state in
+// device memory; a single-workgroup kernel continues it (and falls back to
+// bracketing) only when those launches did not converge.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 
 #include "ldga_internal.h"
+#include "ldga_reduce.h"
 
 namespace {
 
-constexpr int kFT = 1024;  // one workgroup does all scalar reductions
+using ldga_red::block_sum;
+using ldga_red::device_sum;
 
-// sum over the workgroup (all threads must call)
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double (*red)[kFT / 64]) {
-#pragma unroll
-  for (int i = 0; i < NV; ++i)
-    for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_xor(v[i], off);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0)
-#pragma unroll
-    for (int i = 0; i < NV; ++i) red[i][wv] = v[i];
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    double s = 0.0;
-    for (int j = 0; j < kFT / 64; ++j) s += red[i][j];
-    v[i] = s;
-  }
-}
+constexpr int kFT = 1024;     // the single-workgroup fallback of the μ fit
+constexpr int kRB = 256;      // threads per reduction workgroup
+constexpr int kRGrid = 64;    // reduction workgroups (at most)
+constexpr int kSecantSteps = 4;  // secant evaluations launched before the fallback kernel
+
+// state of one residual evaluation, in doubles after the 20 reported values
+enum : int {
+  S_X0 = 24, S_X1, S_F0, S_F1, S_IT, S_DONE, S_STALL, S_MU, S_FMU, S_SM, S_DSM, S_SD, S_DSD, S_CMU,
+  S_END
+};
+enum : int { C_MU0 = 0, C_STEP, C_EN, C_NCNT };  // ticket counters
 
 struct FinalArgs {
   const double2 *sigma;  // Nq x numax
@@ -38,9 +39,12 @@ struct FinalArgs {
   int Nq, numax, Nw, n_iw;
   double Nk, beta, U, n, mu0, shell, lam_m, lam_d, c2m, c2d, tol;
   int fit_mu, want_jac;
-  double *res;   // 16 doubles: n mu Ekin Epot1 Epot2 lhs res_pp res_epot | Sm' Sd' - Fmu Emu it conv n_full
-  double2 *W;    // Nq x numax adjoint weight (want_jac)
-  double2 *Wk;   // numax: sum_k kMult W(k, nu)
+  double *res;      // 20 reported doubles (n mu Ekin Epot1 Epot2 lhs res_pp res_epot | Sm' Sd' - Fmu Emu it conv
+                    // n_full | adjoint contractions), then the state S_*
+  double *part;     // partial sums: kRGrid x 8
+  unsigned *ticket; // C_NCNT ticket counters
+  double2 *W;       // Nq x numax adjoint weight (want_jac)
+  double2 *Wk;      // numax: sum_k kMult W(k, nu)
 };
 
 // G(k, nu) = 1 / (i nu_n + mu - eps_k - Sigma(k, nu))
@@ -48,121 +52,191 @@ __device__ __forceinline__ double2 green(double nu_n, double mu, double eps, dou
   return crecip(cz(mu - eps - sig.x, nu_n - sig.y));
 }
 
-__global__ __launch_bounds__(kFT) void lambdadm_final_kernel(OctTables T, FinalArgs A) {
-  __shared__ double red[8][kFT / 64];
-  const int tid = threadIdx.x;
-  const double beta = A.beta, U = A.U, n = A.n, Nk = A.Nk;
-  const double pib = M_PI / beta;
-  // ---- χ sums: sum_kω(χ^λ) over all ω and their λ derivatives (DataTypes.jl:230-249)
-  double v4[4] = {0.0, 0.0, 0.0, 0.0};
-  const bool have_chi = A.chim != nullptr && A.chid != nullptr;  // NULL: G and energies only
-  const size_t tot = have_chi ? (size_t)A.Nq * A.Nw : 0;
-  for (size_t i = tid; i < tot; i += kFT) {
-    const int q = (int)(i % A.Nq);
-    const double km = T.kmult[q];
-    double xm = A.chim[i], xd = A.chid[i];
-    if (A.lam_m != 0.0) xm = xm / (A.lam_m * xm + 1.0);
-    if (A.lam_d != 0.0) xd = xd / (A.lam_d * xd + 1.0);
-    v4[0] = fma(km, xm, v4[0]);
-    v4[1] = fma(km, -xm * xm, v4[1]);
-    v4[2] = fma(km, xd, v4[2]);
-    v4[3] = fma(km, -xd * xd, v4[3]);
+// filling_pos with the shell correction (GFTools.jl:255-268) from Re sum_k G / Nk
+__device__ __forceinline__ double filling_of(const FinalArgs &A, double mu, double reSG) {
+  return 2.0 * (2.0 * reSG / A.beta + 0.5 + mu * A.shell) / (1.0 + A.U * A.shell);
+}
+
+// χ sums over all ω with their λ derivatives (DataTypes.jl:230-249), and the
+// first two filling evaluations of the secant, f(mu0) and f(mu0 (1 + 1e-4))
+__global__ __launch_bounds__(kRB) void ldm_start_kernel(OctTables T, FinalArgs A) {
+  const double pib = M_PI / A.beta;
+  const double x0 = A.mu0, x1 = (x0 == 0.0) ? 1e-4 : x0 * (1.0 + 1e-4);
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  const int stride = gridDim.x * blockDim.x, t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (A.chim && A.chid) {
+    const int tot = A.Nq * A.Nw;
+    for (int i = t0; i < tot; i += stride) {
+      const double km = T.kmult[i % A.Nq];
+      double xm = A.chim[i], xd = A.chid[i];
+      if (A.lam_m != 0.0) xm = xm / (A.lam_m * xm + 1.0);
+      if (A.lam_d != 0.0) xd = xd / (A.lam_d * xd + 1.0);
+      v[0] = fma(km, xm, v[0]);
+      v[1] = fma(km, -xm * xm, v[1]);
+      v[2] = fma(km, xd, v[2]);
+      v[3] = fma(km, -xd * xd, v[3]);
+    }
   }
-  block_sum<4>(v4, red);
+  if (A.fit_mu) {
+    const int ne = A.Nq * A.numax;
+    for (int i = t0; i < ne; i += stride) {
+      const int q = (int)(i % A.Nq), nu = (int)(i / A.Nq);
+      const double nun = (2.0 * nu + 1.0) * pib, km = T.kmult[q], e = T.eps[q];
+      const double2 sig = A.sigma[i];
+      v[4] = fma(km, green(nun, x0, e, sig).x, v[4]);
+      v[5] = fma(km, green(nun, x1, e, sig).x, v[5]);
+    }
+  }
+  double tot[6];
+  if (!device_sum<6>(v, A.part, A.ticket + C_MU0, tot) || threadIdx.x != 0) return;
+  double *r = A.res;
   // the 1/(i w_m)^2 tail over all w per unit c2 (DataTypes.jl:223-225); each sum has its own c2
+  const double beta = A.beta;
   double tail1 = 0.0;
   for (int w = 0; w < A.Nw; ++w) {
     const int m = w - A.n_iw;
     if (m != 0) tail1 += -beta * beta / (4.0 * M_PI * M_PI * (double)m * (double)m);
   }
-  const double Sm = (v4[0] / Nk - A.c2m * tail1 - beta * beta / 12.0 * A.c2m) / beta, dSm = v4[1] / Nk / beta;
-  const double Sd = (v4[2] / Nk - A.c2d * tail1 - beta * beta / 12.0 * A.c2d) / beta, dSd = v4[3] / Nk / beta;
+  r[S_SM] = (tot[0] / A.Nk - A.c2m * tail1 - beta * beta / 12.0 * A.c2m) / beta;
+  r[S_DSM] = tot[1] / A.Nk / beta;
+  r[S_SD] = (tot[2] / A.Nk - A.c2d * tail1 - beta * beta / 12.0 * A.c2d) / beta;
+  r[S_DSD] = tot[3] / A.Nk / beta;
+  r[S_IT] = 0.0;
+  r[S_STALL] = 0.0;
+  if (!A.fit_mu) {
+    r[S_MU] = A.mu0;
+    r[S_FMU] = 0.0;
+    r[S_DONE] = 1.0;
+    return;
+  }
+  const double f0 = filling_of(A, x0, tot[4] / A.Nk) - A.n, f1 = filling_of(A, x1, tot[5] / A.Nk) - A.n;
+  r[S_X0] = x0;
+  r[S_X1] = x1;
+  r[S_F0] = f0;
+  r[S_F1] = f1;
+  r[S_MU] = x1;
+  r[S_FMU] = f1;
+  // the checks at the top of the secant loop (GFTools.jl:172 via Roots.find_zero)
+  r[S_DONE] = fabs(f1) < A.tol ? 1.0 : 0.0;
+  r[S_STALL] = (r[S_DONE] == 0.0 && f1 == f0) ? 1.0 : 0.0;
+}
 
-  // ---- filling_pos(G[:, 0:numax-1]) (GFTools.jl:255-273) as a function of mu
-  const size_t ne = (size_t)A.Nq * A.numax;
+// one secant step: x2 = x1 - f1 (x1 - x0) / (f1 - f0), f(x2) across the device
+__global__ __launch_bounds__(kRB) void ldm_secant_kernel(OctTables T, FinalArgs A) {
+  double *r = A.res;
+  if (r[S_DONE] != 0.0 || r[S_STALL] != 0.0) return;  // uniform over the grid
+  const double x0 = r[S_X0], x1 = r[S_X1], f0 = r[S_F0], f1 = r[S_F1];
+  const double x2 = x1 - f1 * (x1 - x0) / (f1 - f0);
+  const double pib = M_PI / A.beta;
+  double v[1] = {0.0};
+  const int ne = A.Nq * A.numax, stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ne; i += stride) {
+    const int q = (int)(i % A.Nq), nu = (int)(i / A.Nq);
+    v[0] = fma(T.kmult[q], green((2.0 * nu + 1.0) * pib, x2, T.eps[q], A.sigma[i]).x, v[0]);
+  }
+  double tot[1];
+  if (!device_sum<1>(v, A.part, A.ticket + C_STEP, tot) || threadIdx.x != 0) return;
+  const double f2 = filling_of(A, x2, tot[0] / A.Nk) - A.n;
+  r[S_X0] = x1;
+  r[S_F0] = f1;
+  r[S_X1] = x2;
+  r[S_F1] = f2;
+  r[S_MU] = x2;
+  r[S_FMU] = f2;
+  r[S_IT] += 1.0;
+  r[S_DONE] = fabs(f2) < A.tol ? 1.0 : 0.0;
+  r[S_STALL] = (r[S_DONE] == 0.0 && (f2 == f1 || r[S_IT] >= 100.0)) ? 1.0 : 0.0;
+}
+
+// One workgroup: the rest of the secant (to 100 steps), then, as a bracketing
+// solver would, stepping out from mu0 until the filling changes sign and
+// Illinois false position.  Returns at once when the secant has converged.
+__global__ __launch_bounds__(kFT) void ldm_mu_fallback_kernel(OctTables T, FinalArgs A) {
+  __shared__ double red[1][ldga_red::kMaxWaves];
+  double *r = A.res;
+  if (r[S_DONE] != 0.0) return;
+  const int tid = threadIdx.x;
+  const double pib = M_PI / A.beta, n = A.n;
+  const int ne = A.Nq * A.numax;
   auto sumG = [&](double mu) -> double {
     double v[1] = {0.0};
-    for (size_t i = tid; i < ne; i += kFT) {
+    for (int i = tid; i < ne; i += kFT) {
       const int q = (int)(i % A.Nq), nu = (int)(i / A.Nq);
-      const double2 g = green((2.0 * nu + 1.0) * pib, mu, T.eps[q], A.sigma[i]);
-      v[0] = fma(T.kmult[q], g.x, v[0]);
+      v[0] = fma(T.kmult[q], green((2.0 * nu + 1.0) * pib, mu, T.eps[q], A.sigma[i]).x, v[0]);
     }
     block_sum<1>(v, red);
-    return v[0] / Nk;
+    return v[0] / A.Nk;
   };
-  auto filling = [&](double mu, double reSG) {
-    return 2.0 * (2.0 * reSG / beta + 0.5 + mu * A.shell) / (1.0 + U * A.shell);
-  };
-  // secant from mu0 (the reference calls Roots.find_zero, GFTools.jl:172)
-  double mu = A.mu0;
-  int iters = 0, conv = 1;
-  if (A.fit_mu) {
-    double x0 = A.mu0, x1 = (x0 == 0.0) ? 1e-4 : x0 * (1.0 + 1e-4);
-    double f0 = filling(x0, sumG(x0)) - n, f1 = filling(x1, sumG(x1)) - n;
-    double fmu = f1;  // f at the returned mu
-    conv = 0;
-    for (iters = 0; iters < 100; ++iters) {
-      if (fabs(f1) < A.tol) {
-        conv = 1;
-        break;
-      }
-      if (f1 == f0) break;
-      const double x2 = x1 - f1 * (x1 - x0) / (f1 - f0);
-      x0 = x1;
-      f0 = f1;
-      x1 = x2;
-      f1 = filling(x1, sumG(x1)) - n;
+  double x0 = r[S_X0], x1 = r[S_X1], f0 = r[S_F0], f1 = r[S_F1];
+  int iters = (int)r[S_IT];
+  bool conv = false;
+  for (; iters < 100; ++iters) {
+    if (fabs(f1) < A.tol) {
+      conv = true;
+      break;
     }
-    mu = x1;
-    fmu = f1;
-    if (!conv) {
-      // fallback, as a bracketing solver would (the filling grows with mu): step
-      // out from mu0 until the sign changes, then Illinois false position
-      double a = A.mu0, fa = filling(a, sumG(a)) - n;
-      const double dir = fa < 0.0 ? 1.0 : -1.0;
-      double h = 1.0, b = a + dir * h, fb = filling(b, sumG(b)) - n;
-      for (int k = 0; k < 60 && !(fa * fb <= 0.0); ++k) {
-        a = b;
-        fa = fb;
-        h *= 2.0;
-        b = a + dir * h;
-        fb = filling(b, sumG(b)) - n;
-      }
-      if (fa * fb <= 0.0) {
-        int side = 0;
-        for (int k = 0; k < 200; ++k, ++iters) {
-          const double c = (a * fb - b * fa) / (fb - fa);
-          const double fc = filling(c, sumG(c)) - n;
-          mu = c;
-          fmu = fc;
-          if (fabs(fc) < A.tol || fabs(b - a) <= 4e-16 * fmax(fabs(a), fabs(b))) {
-            conv = 1;
-            break;
-          }
-          if (fc * fb > 0.0) {
-            b = c;
-            fb = fc;
-            if (side == -1) fa *= 0.5;
-            side = -1;
-          } else {
-            a = c;
-            fa = fc;
-            if (side == 1) fb *= 0.5;
-            side = 1;
-          }
+    if (f1 == f0) break;
+    const double x2 = x1 - f1 * (x1 - x0) / (f1 - f0);
+    x0 = x1;
+    f0 = f1;
+    x1 = x2;
+    f1 = filling_of(A, x1, sumG(x1)) - n;
+  }
+  double mu = x1, fmu = f1;
+  if (!conv) {
+    double a = A.mu0, fa = filling_of(A, a, sumG(a)) - n;
+    const double dir = fa < 0.0 ? 1.0 : -1.0;
+    double h = 1.0, b = a + dir * h, fb = filling_of(A, b, sumG(b)) - n;
+    for (int k = 0; k < 60 && !(fa * fb <= 0.0); ++k) {
+      a = b;
+      fa = fb;
+      h *= 2.0;
+      b = a + dir * h;
+      fb = filling_of(A, b, sumG(b)) - n;
+    }
+    if (fa * fb <= 0.0) {
+      int side = 0;
+      for (int k = 0; k < 200; ++k, ++iters) {
+        const double c = (a * fb - b * fa) / (fb - fa);
+        const double fc = filling_of(A, c, sumG(c)) - n;
+        mu = c;
+        fmu = fc;
+        if (fabs(fc) < A.tol || fabs(b - a) <= 4e-16 * fmax(fabs(a), fabs(b))) break;
+        if (fc * fb > 0.0) {
+          b = c;
+          fb = fc;
+          if (side == -1) fa *= 0.5;
+          side = -1;
+        } else {
+          a = c;
+          fa = fc;
+          if (side == 1) fb *= 0.5;
+          side = 1;
         }
       }
     }
-    // a fit that did not reach the reference's tolerance (find_zero atol = 1e-8,
-    // GFTools.jl:171-176) fails: mu = NaN, and every result below is NaN
-    conv = fabs(fmu) <= 1e-8;
-    if (!conv) mu = NAN;
   }
-  // ---- filling, calc_E (thermodynamics.jl:142-168), and for the Jacobian
-  //      d(filling)/dmu and dE_pot/dmu
+  __syncthreads();  // every thread has read the state before it is written
+  if (tid == 0) {
+    r[S_MU] = mu;
+    r[S_FMU] = fmu;
+    r[S_IT] = (double)iters;
+  }
+}
+
+// filling, calc_E (thermodynamics.jl:142-168) and, for the Jacobian,
+// d(filling)/dmu and dE_pot/dmu; the last workgroup writes the report.  A fit
+// that did not reach the reference's tolerance (find_zero atol = 1e-8,
+// GFTools.jl:171-176) fails: mu = NaN, and every result is NaN.
+__global__ __launch_bounds__(kRB) void ldm_energies_kernel(OctTables T, FinalArgs A) {
+  double *r = A.res;
+  const bool conv = !A.fit_mu || fabs(r[S_FMU]) <= 1e-8;
+  const double mu = conv ? r[S_MU] : NAN;
+  const double beta = A.beta, U = A.U, n = A.n, pib = M_PI / beta;
   const double Sh = n * U / 2.0;
-  double v6[6] = {0, 0, 0, 0, 0, 0};  // Re sG, Ekin, Epot, Re sum G^2 (dF/dmu), dEpot/dmu
-  for (size_t i = tid; i < ne; i += kFT) {
+  double v[5] = {0, 0, 0, 0, 0};  // Re sG, Ekin, Epot, Re sum -G^2 (dF/dmu), dEpot/dmu
+  const int ne = A.Nq * A.numax, stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ne; i += stride) {
     const int q = (int)(i % A.Nq), nu = (int)(i / A.Nq);
     const double e = T.eps[q], km = T.kmult[q];
     const double nun = (2.0 * nu + 1.0) * pib;
@@ -173,67 +247,71 @@ __global__ __launch_bounds__(kFT) void lambdadm_final_kernel(OctTables T, FinalA
     const double Ept = U * U * 0.5 * n * (1.0 - 0.5 * n) + Sh * (e + Sh - mu);
     const double2 gs = cmul(g, sig);
     const double2 g2 = cmul(g, g);
-    v6[0] = fma(km, g.x, v6[0]);
-    v6[1] = fma(km, 4.0 * e * (g.x - Ekt / inu2), v6[1]);
-    v6[2] = fma(km, 2.0 * (gs.x - Ept / inu2), v6[2]);
-    v6[3] = fma(km, -g2.x, v6[3]);
-    v6[4] = fma(km, 2.0 * (-cmul(g2, sig).x + Sh / inu2), v6[4]);
+    v[0] = fma(km, g.x, v[0]);
+    v[1] = fma(km, 4.0 * e * (g.x - Ekt / inu2), v[1]);
+    v[2] = fma(km, 2.0 * (gs.x - Ept / inu2), v[2]);
+    v[3] = fma(km, -g2.x, v[3]);
+    v[4] = fma(km, 2.0 * (-cmul(g2, sig).x + Sh / inu2), v[4]);
     if (nu == 0) {  // the per-k tail terms, once per k
-      v6[1] = fma(km, (beta / 2.0) * e * (1.0 - beta * Ekt), v6[1]);
-      v6[2] = fma(km, (beta / 2.0) * Sh - (beta * beta / 4.0) * Ept, v6[2]);
-      v6[4] = fma(km, beta * beta * Sh / 4.0, v6[4]);
+      v[1] = fma(km, (beta / 2.0) * e * (1.0 - beta * Ekt), v[1]);
+      v[2] = fma(km, (beta / 2.0) * Sh - (beta * beta / 4.0) * Ept, v[2]);
+      v[4] = fma(km, beta * beta * Sh / 4.0, v[4]);
     }
   }
-  block_sum<6>(v6, red);
-  const double nval = filling(mu, v6[0] / Nk);
-  const double Ekin = v6[1] / Nk / beta, Epot1 = v6[2] / Nk / beta;
+  double tot[5];
+  if (!device_sum<5>(v, A.part, A.ticket + C_EN, tot) || threadIdx.x != 0) return;
+  const double Nk = A.Nk;
+  const bool have_chi = A.chim != nullptr && A.chid != nullptr;
+  const double Sm = r[S_SM], Sd = r[S_SD];
+  const double nval = filling_of(A, mu, tot[0] / Nk);
+  const double Ekin = tot[1] / Nk / beta, Epot1 = tot[2] / Nk / beta;
   const double lhs = have_chi ? 0.5 * (Sd + Sm) : NAN;
   const double Epot2 = have_chi ? 0.5 * U * (Sd - Sm) + U * (n / 2.0) * (n / 2.0) : NAN;
-  if (tid == 0) {
-    double *r = A.res;
-    r[0] = nval;
-    r[1] = mu;
-    r[2] = Ekin;
-    r[3] = Epot1;
-    r[4] = Epot2;
-    r[5] = lhs;
-    r[6] = lhs - n / 2.0 * (1.0 - n / 2.0);
-    r[7] = Epot1 - Epot2;
-    r[8] = dSm;
-    r[9] = dSd;
-    r[13] = (double)iters;
-    r[14] = (double)conv;
-    r[15] = NAN;  // the full filling, written by nfull_kernel when G_ladder is formed
+  r[0] = nval;
+  r[1] = mu;
+  r[2] = Ekin;
+  r[3] = Epot1;
+  r[4] = Epot2;
+  r[5] = lhs;
+  r[6] = lhs - n / 2.0 * (1.0 - n / 2.0);
+  r[7] = Epot1 - Epot2;
+  r[8] = r[S_DSM];
+  r[9] = r[S_DSD];
+  r[13] = r[S_IT];
+  r[14] = conv ? 1.0 : 0.0;
+  r[15] = NAN;  // the full filling, written by nfull_kernel when G_ladder is formed
+  const double Fmu = 2.0 * (2.0 * (tot[3] / Nk) / beta + A.shell) / (1.0 + U * A.shell);
+  const double Emu = tot[4] / Nk / beta;
+  r[11] = Fmu;
+  r[12] = Emu;
+  r[S_CMU] = -(Emu / Fmu) * 4.0 / (beta * (1.0 + U * A.shell));
+}
+
+// adjoint weight W = dR_epot/dSigma (a full-grid function, wedge values), elementwise
+__global__ __launch_bounds__(kRB) void ldm_adjw_kernel(OctTables T, FinalArgs A) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.Nq * A.numax) return;
+  const double mu = A.res[1], cmu = A.res[S_CMU], beta = A.beta;
+  const int q = (int)(i % A.Nq), nu = (int)(i / A.Nq);
+  const double2 sig = A.sigma[i];
+  const double2 g = green((2.0 * nu + 1.0) * M_PI / beta, mu, T.eps[q], sig);
+  const double2 g2 = cmul(g, g);
+  const double2 wv = cadd(cscale(cadd(g, cmul(sig, g2)), 2.0 / beta), cscale(g2, cmu));
+  A.W[i] = cscale(wv, 1.0 / A.Nk);
+}
+
+// Wk[nu] = sum_k kMult W(k, nu): one workgroup per nu
+__global__ __launch_bounds__(kRB) void ldm_wk_kernel(OctTables T, FinalArgs A) {
+  __shared__ double red[2][ldga_red::kMaxWaves];
+  const int nu = blockIdx.x;
+  double v[2] = {0.0, 0.0};
+  for (int q = threadIdx.x; q < A.Nq; q += blockDim.x) {
+    const double2 w = A.W[q + (size_t)A.Nq * nu];
+    v[0] = fma(T.kmult[q], w.x, v[0]);
+    v[1] = fma(T.kmult[q], w.y, v[1]);
   }
-  if (!A.want_jac) return;
-  // ---- adjoint weight W = dR_epot/dSigma (full-grid function, wedge values)
-  const double Fmu = 2.0 * (2.0 * (v6[3] / Nk) / beta + A.shell) / (1.0 + U * A.shell);
-  const double Emu = v6[4] / Nk / beta;
-  const double cmu = -(Emu / Fmu) * 4.0 / (beta * (1.0 + U * A.shell));
-  for (size_t i = tid; i < ne; i += kFT) {
-    const int q = (int)(i % A.Nq), nu = (int)(i / A.Nq);
-    const double2 sig = A.sigma[i];
-    const double2 g = green((2.0 * nu + 1.0) * pib, mu, T.eps[q], sig);
-    const double2 g2 = cmul(g, g);
-    const double2 wv = cadd(cscale(cadd(g, cmul(sig, g2)), 2.0 / beta), cscale(g2, cmu));
-    A.W[i] = cscale(wv, 1.0 / Nk);
-  }
-  // sum_k kMult W(k, nu), recomputed per nu (no read-back of W across threads)
-  for (int nu = tid; nu < A.numax; nu += kFT) {
-    double2 s = cz(0.0, 0.0);
-    for (int q = 0; q < A.Nq; ++q) {
-      const double2 sig = A.sigma[q + (size_t)A.Nq * nu];
-      const double2 g = green((2.0 * nu + 1.0) * pib, mu, T.eps[q], sig);
-      const double2 g2 = cmul(g, g);
-      const double2 wv = cadd(cscale(cadd(g, cmul(sig, g2)), 2.0 / beta), cscale(g2, cmu));
-      s = cadd(s, cscale(wv, T.kmult[q] / Nk));
-    }
-    A.Wk[nu] = s;
-  }
-  if (tid == 0) {
-    A.res[11] = Fmu;
-    A.res[12] = Emu;
-  }
+  block_sum<2>(v, red);
+  if (threadIdx.x == 0) A.Wk[nu] = cz(v[0], v[1]);
 }
 
 // sigma <- (1 - m) sigma + m sigma_prev (conditions_new.jl:484)
@@ -329,9 +407,11 @@ static int lambdadm_dev(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp
     for (double &j : out->jac) j = nan;
     return LDGA_OK;
   }
-  double2 *sigma = sigma_out ? sigma_out : (double2 *)ldga_buf(ctx, "ldm_sigma", (size_t)Nq * numax * 16);
-  double *res = (double *)ldga_buf(ctx, "ldm_res", 16 * 8 + 2 * 16);
-  if (!sigma || !res) return LDGA_ERR_HIP;
+  double2 *sigma = sigma_out ? sigma_out : (double2 *)ldga_scratch(ctx, "ldm_sigma", (size_t)Nq * numax * 16, st);
+  double *res = (double *)ldga_scratch(ctx, "ldm_res", S_END * 8, st);
+  double *part = (double *)ldga_scratch(ctx, "ldm_part", (size_t)kRGrid * 8 * 8, st);
+  unsigned *ticket = ldga_tickets(ctx, "ldm_ticket", C_NCNT, st);
+  if (!sigma || !res || !part || !ticket) return LDGA_ERR_HIP;
   int rc = LDGA_OK;
   if (eom) {
     rc = launch_sigma(ctx, fr, mp, numax, lam_m, lam_d, tc, LDGA_EOM_FULL, chiloc_m_sum, chi_m, gamma_m, chi_d,
@@ -364,13 +444,28 @@ static int lambdadm_dev(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp
   A.fit_mu = fit_mu;
   A.want_jac = want_jac;
   A.res = res;
+  A.part = part;
+  A.ticket = ticket;
   double2 *adj = (double2 *)(res + 16);
   if (want_jac) {
-    A.W = (double2 *)ldga_buf(ctx, "ldm_W", (size_t)Nq * numax * 16);
-    A.Wk = (double2 *)ldga_buf(ctx, "ldm_Wk", (size_t)numax * 16);
+    A.W = (double2 *)ldga_scratch(ctx, "ldm_W", (size_t)Nq * numax * 16, st);
+    A.Wk = (double2 *)ldga_scratch(ctx, "ldm_Wk", (size_t)numax * 16, st);
     if (!A.W || !A.Wk) return LDGA_ERR_HIP;
   }
-  lambdadm_final_kernel<<<1, kFT, 0, st>>>(ctx->oct, A);
+  // the reductions: enough workgroups for one element per thread, at most kRGrid
+  const size_t ne = (size_t)Nq * numax, nchi = (size_t)Nq * Nw;
+  const unsigned g_fit = (unsigned)std::min<size_t>(kRGrid, (std::max(ne, nchi) + kRB - 1) / kRB);
+  const unsigned g_ne = (unsigned)std::min<size_t>(kRGrid, (ne + kRB - 1) / kRB);
+  ldm_start_kernel<<<g_fit, kRB, 0, st>>>(ctx->oct, A);
+  if (fit_mu) {
+    for (int k = 0; k < kSecantSteps; ++k) ldm_secant_kernel<<<g_ne, kRB, 0, st>>>(ctx->oct, A);
+    ldm_mu_fallback_kernel<<<1, kFT, 0, st>>>(ctx->oct, A);
+  }
+  ldm_energies_kernel<<<g_ne, kRB, 0, st>>>(ctx->oct, A);
+  if (want_jac) {
+    ldm_adjw_kernel<<<(unsigned)((ne + kRB - 1) / kRB), kRB, 0, st>>>(ctx->oct, A);
+    ldm_wk_kernel<<<(unsigned)numax, kRB, 0, st>>>(ctx->oct, A);
+  }
   LDGA_HIP_TRY(ctx, hipGetLastError());
   if (want_jac) {
     rc = launch_eom_adjoint(ctx, fr, mp, numax, lam_m, lam_d, chi_m, gamma_m, chi_d, gamma_d, Grfft, A.W, adj, st);
@@ -379,7 +474,7 @@ static int lambdadm_dev(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp
   if (G_out) {
     const size_t tot = (size_t)Nq * fr->n_fft;
     const unsigned nb = (unsigned)((tot + kGL - 1) / kGL);
-    double *part = (double *)ldga_buf(ctx, "ldm_gl_part", (size_t)nb * 8);
+    double *part = (double *)ldga_scratch(ctx, "ldm_gl_part", (size_t)nb * 8, st);
     if (!part) return LDGA_ERR_HIP;
     gladder_kernel<<<nb, kGL, 0, st>>>(ctx->oct, numax, fr->fft_first, fr->n_fft, mp->beta, res, sigma, sigloc, G_out,
                                        part);

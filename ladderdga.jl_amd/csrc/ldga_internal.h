@@ -41,6 +41,7 @@ struct OctTables {
   const int32_t *octfull;    // NO: linear index of the octant point in the full Ns^D grid
   const double *octw;        // NO: multiplicity weight prod_d w(R_d), w = 1 at 0 and Ns/2, else 2
   const double2 *octphase;   // NO: exp(-2 pi i sum_d R_d / Ns), undoes the reverse() of G_rfft
+  const double2 *octpw;      // NO: octphase * octw
   const double *cosT;        // Ns: cos(2 pi m / Ns)
   const double *kmult;       // Nq: wedge multiplicities
   const double *eps;         // Nq: dispersion on the wedge
@@ -61,6 +62,7 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nblocks) {
 struct DevBuf {
   void *ptr = nullptr;
   size_t bytes = 0;
+  bool fresh = false;  // allocated since its last use as a ticket buffer
 };
 
 struct ldga_ctx {
@@ -78,6 +80,9 @@ struct ldga_ctx {
   // device address of this device's copy of the allocation-free entry
   // point's static scratch array (set by ldga_create)
   void *noalloc_scratch = nullptr;
+  // resident eom_accum workgroups on the device, for the grid size it was found for
+  int eom_resident = 0;
+  size_t eom_resident_key = 0;
 };
 
 // looks up the static scratch array of the allocation-free entry point on the
@@ -93,8 +98,12 @@ int ldga_fail(ldga_ctx *ctx, int code, const char *fmt, ...);
                                            __FILE__, __LINE__);                                             \
   } while (0)
 
-// cached device scratch (returns nullptr on failure and sets ctx->err)
+// cached device buffers (returns nullptr on failure and sets ctx->err): ldga_buf
+// for the host-array forms' staging, ldga_scratch for the launchers' scratch
+// (one per role and stream), ldga_tickets for device_sum's ticket counters
 void *ldga_buf(ldga_ctx *ctx, const char *role, size_t bytes);
+void *ldga_scratch(ldga_ctx *ctx, const char *role, size_t bytes, hipStream_t st);
+unsigned *ldga_tickets(ldga_ctx *ctx, const char *role, int n, hipStream_t st);
 hipStream_t ldga_stream(ldga_ctx *ctx, void *stream);
 int ldga_check_freq(ldga_ctx *ctx, const ldga_freq *fr);
 
@@ -119,7 +128,7 @@ int launch_gen_chi(ldga_ctx *ctx, const ldga_freq *fr, const double2 *Gam, const
 int launch_sum_w(ldga_ctx *ctx, const double *chi, int Nq, int Nw, int w_lo, int w_hi, double c2, double beta,
                  double lam, double *res, double *out_k, hipStream_t st);
 // EoM partial sums: one (nu, omega-chunk) workgroup each; returns the chunking used
-int eom_chunks(int Nw, int numax, int *wchunk);
+int eom_chunks(ldga_ctx *ctx, int Nw, int numax, int *wchunk);
 int launch_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m, double lam_d,
                  int tc, int eom_kind, double chiloc_m_sum, const double *chi_m, const double2 *gamma_m,
                  const double *chi_d, const double2 *gamma_d, const double2 *lambda0, const double2 *Grfft,
