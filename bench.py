@@ -145,8 +145,9 @@ def lambdadm_step_costs(L, ctx, o, λ0, inp, numax, Nq, Nν, Nω, reps=5):
 def run_sc_iteration_cost(L, ctx, o, λ0, inp, numax, Nq, Nν, Nω):
     """Cost of one iteration of the self-consistency loop (run_sc!, conditions_new.jl:479-491) on this
     step's ladder arrays: EoM, mixing, μ fit and energies, G-table rebuild, all resident on the device.
-    Taken as (t(5 iterations) - t(1 iteration)) / 4 of run_sc type :fix, so the one-time uploads
-    cancel; conv_abs = 0 keeps the loop from stopping early."""
+    Taken as (median t(9 iterations) - median t(1 iteration)) / 8 over 7 calls each of run_sc type :fix,
+    so the one-time uploads and their host-side jitter cancel; conv_abs = 0 keeps the loop from
+    stopping early."""
     import numpy as np
     Fo = lambda a, shape: np.asfortranarray(a.cpu().numpy().reshape(shape, order="F"))
     χm = L.χT(Fo(o["χm"], (Nq, Nω)), inp.mP.β, tail_c=[0, 0, inp.mP.Ekin_DMFT])
@@ -164,8 +165,14 @@ def run_sc_iteration_cost(L, ctx, o, λ0, inp, numax, Nq, Nν, Nω):
         return time.perf_counter() - t0, maxit if np.isfinite(r.μ) else 0
 
     t(1)
-    (t1, n1), (t5, n5) = t(1), t(5)
-    return {"iteration_ms": (t5 - t1) / max(n5 - n1, 1) * 1e3, "iterations_timed": n5 - n1, "lambda": list(lam)}
+    t(9)
+    r1 = [t(1) for _ in range(7)]
+    r9 = [t(9) for _ in range(7)]
+    n1, n9 = min(n for _, n in r1), min(n for _, n in r9)
+    t1 = float(np.median([x for x, _ in r1]))
+    t9 = float(np.median([x for x, _ in r9]))
+    return {"iteration_ms": (t9 - t1) / max(n9 - n1, 1) * 1e3, "iterations_timed": n9 - n1,
+            "one_iteration_call_ms": t1 * 1e3, "lambda": list(lam)}
 
 
 def main():
@@ -300,6 +307,18 @@ def main():
         "mixed_vs_fp64_gpu": (ms_other / ms_main) if main_prec == "mixed" else (ms_main / ms_other),
         "mixed_rel_err_vs_fp64": errs,
     }
+    # refinement statistics of the timed mixed BSE (one extra call per channel with a report)
+    bse_rep = {}
+    for s, Γ, ch in ((1, Γm, "m"), (-1, Γd, "d")):
+        rp = _lib.BseReport()
+        o = out["mixed"]
+        ctx.check(Lb.ldga_chi_gamma_dev(ctx.handle, s, _lib.PREC_MIXED, frp, mP.U, mP.β, Γ.data_ptr(),
+                                        χ0.data_ptr(), o["χ" + ch].data_ptr(), o["γ" + ch].data_ptr(),
+                                        C.byref(rp), sp))
+        bse_rep[ch] = {"ir_steps_mean": rp.ir_steps_mean, "ir_steps_max": rp.ir_steps_max,
+                       "n_fallback": rp.n_fallback, "n_fp64_resolved": rp.n_fp64_resolved,
+                       "max_rel_resid": rp.max_rel_resid}
+    result["mixed_bse_report"] = bse_rep
     result["lambdadm_newton_step"] = lambdadm_step_costs(L, ctx, out[main_prec], λ0, inp, numax, Nq, Nν, Nω)
     result["run_sc_iteration"] = run_sc_iteration_cost(L, ctx, out[main_prec], λ0, inp, numax, Nq, Nν, Nω)
     # G tables (helpers.jl:153-183) from the wedge G_loc, as the self-consistency loop rebuilds them:

@@ -56,8 +56,13 @@ struct BlkShared {
   // rows padded so that rows 1024 bytes apart (NC = 128) do not fall on the
   // same LDS banks (the column reads of S and the per-group reads of W)
   static constexpr int SPAD = sizeof(T) == 8 ? 4 : 2, WPAD = sizeof(T) == 8 ? 32 : 16;
+  static constexpr bool BF = std::is_same<T, float>::value;
   c2 W[3][4][NC + WPAD];   // published panel update (triple-buffered)
-  c2 S[3][16][NC + SPAD];  // a panel owner's 16 rows, staged (triple-buffered)
+  // FP32: W as the bf16 MFMA B operand (bf_pack_b), published with it
+  u32x4 Wb[BF ? 3 : 1][4][BF ? NC + 1 : 1];
+  // staged by a panel's owner (triple-buffered): FP32 the panel's four rows
+  // (row r: lane group r), FP64 all sixteen rows of the owner
+  c2 S[3][BF ? 4 : 16][NC + SPAD];
   T Cr[NT][16][4];     // per wave: own rows at the four pivot columns (real)
   T Ci[NT][16][4];     // (imaginary)
   int P[3][4];         // pivot columns of the panel
@@ -200,14 +205,25 @@ __device__ __forceinline__ void blk_solve(BlkShared<T, NT> &sh, const int lb, in
   // keeping (and spilling) one hoisted register per array element.
   int Lg = g, Ln = n, Ll = lane;
 
-  // all 16 rows of this wave (the owner of panel m) -> staging buffer m % 3
+  // the rows of this wave (the owner of panel m) -> staging buffer m % 3:
+  // FP32 the four panel rows (element m / NT of each lane group; row r = lane
+  // group r), FP64 all sixteen (tile row order)
   auto stage_rows = [&](int m) {
     const int sb = m % 3;
+    if constexpr (!std::is_same<T, float>::value) {
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
+      for (int t = 0; t < NT; ++t)
 #pragma unroll
-      for (int jj = 0; jj < 4; ++jj)
-        sh.S[sb][MT::trow(Lg, jj)][16 * t + Ln] = mkc<T>(Vr.get(t, jj), Vi.get(t, jj));
+        for (int jj = 0; jj < 4; ++jj)
+          sh.S[sb][MT::trow(Lg, jj)][16 * t + Ln] = mkc<T>(Vr.get(t, jj), Vi.get(t, jj));
+      return;
+    }
+    auto body = [&](auto J) {
+#pragma unroll
+      for (int t = 0; t < NT; ++t) sh.S[sb][Lg][16 * t + Ln] = mkc<T>(Vr.get(t, J), Vi.get(t, J));
+    };
+    if constexpr (decltype(Vr)::bank) body(__builtin_amdgcn_readfirstlane(m / NT));
+    else with_j(m / NT, body);
   };
 
   // this wave's rows at the pivot columns q0..q3 -> Cr/Ci[wave] (the A operand
@@ -226,45 +242,74 @@ __device__ __forceinline__ void blk_solve(BlkShared<T, NT> &sh, const int lb, in
       }
     }
   };
-  // tile t += A W[wb]: four K=4 MFMAs (a: tile row n, k = g)
-  auto mma_tile = [&](int t, c2 a, int wb) {
-    const c2 bb = sh.W[wb][Lg][16 * t + Ln];  // B operand: k = g, column 16t + n
+  // tile t += A W[wb] (a: tile row n, k = g).  FP64: four K=4 f64 MFMAs.
+  // FP32: the A operand split once per panel (prep), the B operand as
+  // published (Wb), four bf16 MFMAs per tile (bf_cmma).
+  constexpr bool kBf = std::is_same<T, float>::value;
+  typedef typename std::conditional<kBf, BfA, c2>::type AOp;
+  auto prep = [&](c2 a) -> AOp {
+    if constexpr (kBf) return bf_prep_a(a.x, a.y);
+    else return a;
+  };
+  auto mma_tile = [&](int t, const AOp &a, int wb) {
     v4 cr = Vr.tile(t), ci = Vi.tile(t);
-    cr = MT::mma(a.x, bb.x, cr);
-    cr = MT::mma(-a.y, bb.y, cr);
-    ci = MT::mma(a.x, bb.y, ci);
-    ci = MT::mma(a.y, bb.x, ci);
+    if constexpr (kBf) {
+      bf_cmma(a, sh.Wb[wb][Lg][16 * t + Ln], cr, ci);  // B operand: k = g, column 16t + n
+    } else {
+      const c2 bb = sh.W[wb][Lg][16 * t + Ln];
+      cr = MT::mma(a.x, bb.x, cr);
+      cr = MT::mma(-a.y, bb.y, cr);
+      ci = MT::mma(a.x, bb.y, ci);
+      ci = MT::mma(a.y, bb.x, ci);
+    }
     Vr.set_tile(t, cr);
     Vi.set_tile(t, ci);
   };
   // the owner of panel kp, after factoring it: its tile += A1 W[w1] + A2 W[w2]
-  // for panels kp-1 (pivots p, deferred) and kp (pivots q), with both A
-  // operands from its staged rows S[sb] (taken after panel kp-2):
-  // A1 = S[:, P], A2 = S[:, Q] + A1 W[w1][:, Q].  (The panel's own rows of the
-  // tile are overwritten with their final values afterwards.)
+  // for panels kp-1 (pivots p, deferred) and kp (pivots q).  Its rows are as
+  // of panel kp-2 (it factored panel kp meanwhile): in its accumulators, and
+  // for FP64 also staged in S[sb].  A1 = M[:, P], A2 = M[:, Q] + A1 W[w1][:, Q].
+  // (The panel's own rows of the tile are overwritten with their final values
+  // afterwards.)
   auto update_tile2 = [&](int sb, int p0, int p1, int p2, int p3, int w1, int q0, int q1, int q2, int q3, int w2) {
-    const int pg = Lg == 0 ? p0 : Lg == 1 ? p1 : Lg == 2 ? p2 : p3;
     const int qg = Lg == 0 ? q0 : Lg == 1 ? q1 : Lg == 2 ? q2 : q3;
-    const c2 a1 = sh.S[sb][Ln][pg];
-    c2 a2 = sh.S[sb][Ln][qg];
+    c2 a1, a2, cp[4];
+    if constexpr (kBf) {  // the FP32 accumulators are read by VGPR indexing
+      gather_cols(p0, p1, p2, p3);
+      wave_sync_lds();
+      a1 = mkc<T>(sh.Cr[wave][Ln][Lg], sh.Ci[wave][Ln][Lg]);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) cp[r] = mkc<T>(sh.Cr[wave][Ln][r], sh.Ci[wave][Ln][r]);
+      wave_sync_lds();
+      gather_cols(q0, q1, q2, q3);
+      wave_sync_lds();
+      a2 = mkc<T>(sh.Cr[wave][Ln][Lg], sh.Ci[wave][Ln][Lg]);
+    } else {
+      const int pg = Lg == 0 ? p0 : Lg == 1 ? p1 : Lg == 2 ? p2 : p3;
+      a1 = sh.S[sb][Ln][pg];
+      a2 = sh.S[sb][Ln][qg];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) cp[r] = sh.S[sb][Ln][r == 0 ? p0 : r == 1 ? p1 : r == 2 ? p2 : p3];
+    }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const c2 c = sh.S[sb][Ln][r == 0 ? p0 : r == 1 ? p1 : r == 2 ? p2 : p3];
+      const c2 c = cp[r];
       const c2 wv = sh.W[w1][r][qg];
       a2.x = fma(c.x, wv.x, fma(-c.y, wv.y, a2.x));
       a2.y = fma(c.x, wv.y, fma(c.y, wv.x, a2.y));
     }
+    const AOp o1 = prep(a1), o2 = prep(a2);
 #pragma unroll
-    for (int t = 0; t < NT; ++t) mma_tile(t, a1, w1);
+    for (int t = 0; t < NT; ++t) mma_tile(t, o1, w1);
 #pragma unroll
-    for (int t = 0; t < NT; ++t) mma_tile(t, a2, w2);
+    for (int t = 0; t < NT; ++t) mma_tile(t, o2, w2);
     __builtin_amdgcn_wave_barrier();
   };
   // this wave's tile += tile[:, Q] W[wb]
   auto update_tile = [&](int q0, int q1, int q2, int q3, int wb) {
     gather_cols(q0, q1, q2, q3);
     wave_sync_lds();
-    const c2 a = mkc<T>(sh.Cr[wave][Ln][Lg], sh.Ci[wave][Ln][Lg]);
+    const AOp a = prep(mkc<T>(sh.Cr[wave][Ln][Lg], sh.Ci[wave][Ln][Lg]));
 #pragma unroll
     for (int t = 0; t < NT; ++t) mma_tile(t, a, wb);
     __builtin_amdgcn_wave_barrier();
@@ -277,9 +322,8 @@ __device__ __forceinline__ void blk_solve(BlkShared<T, NT> &sh, const int lb, in
     __builtin_amdgcn_s_setprio(3);
     const int sb = kp1 % 3;
     if constexpr (!pre) wave_sync_lds();  // own staging just before (later: a barrier in between)
-    // the panel's rows in the staged copy: element j1 of the four lane groups
-    const int pr0 = MT::trow(0, j1), pr1 = MT::trow(1, j1), pr2 = MT::trow(2, j1), pr3 = MT::trow(3, j1);
-    auto prow = [&](int r) { return r == 0 ? pr0 : r == 1 ? pr1 : r == 2 ? pr2 : pr3; };
+    // the panel's rows in the staged copy (row r: element j1 of lane group r)
+    auto prow = [&](int r) { return kBf ? r : MT::trow(r, j1); };
     // the panel rows from the staged copy, one column per lane
     T pr[NU][4], pim[NU][4];
 #pragma unroll
@@ -415,7 +459,11 @@ __device__ __forceinline__ void blk_solve(BlkShared<T, NT> &sh, const int lb, in
       const int c = lane + 64 * u;
       if (c < NC) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) sh.W[bo][r][Ll + 64 * u] = mkc<T>(pr[u][r] - (c == P[r] ? (T)1 : (T)0), pim[u][r]);
+        for (int r = 0; r < 4; ++r) {
+          const T wx = pr[u][r] - (c == P[r] ? (T)1 : (T)0);
+          sh.W[bo][r][Ll + 64 * u] = mkc<T>(wx, pim[u][r]);
+          if constexpr (kBf) sh.Wb[bo][r][Ll + 64 * u] = bf_pack_b(wx, pim[u][r]);
+        }
       }
     }
     if (lane < 4) {

@@ -139,6 +139,79 @@ __device__ __forceinline__ unsigned wave_max_u32(unsigned k) {
   return k;
 }
 
+// ---- complex rank-4 products for the FP32 path on bf16 MFMA -------------------
+// An FP32 value is split into three bf16 parts, x = h + m + l (each part the
+// round-to-nearest-even bf16 of the remainder; the remainders are exact in FP32
+// and the three parts hold all 24 bits, so h + m + l == x exactly).  A product
+// a b is taken as the sum of seven of the nine part products (all but l*m and
+// l*l, which are below 2^-24 |a b|), each exact in FP32: its error is below
+// that of one FP32 multiplication.  A complex update C += a b (K = 4) becomes
+// two 16x16x32 bf16 MFMAs per plane: lane group g holds the eight K slots of
+// panel column k = g, a dword pairing the real part (low half) with the
+// imaginary part (high half) of one split level.
+//   B operand (both planes, both MFMAs):  [h | m | l | h] of (br, bi)
+//   real plane A:       MFMA 1 [h h h m], MFMA 2 [l m m 0] of (ar, -ai)
+//   imaginary plane A:  the same levels of (ai, ar)
+// i.e. slot pairs (A, B) = (h,h) (h,m) (h,l) (m,h) | (l,h) (m,m) (m,l) (0,h).
+// Unlike the FP32 MFMA, bf16 MFMAs leave the VALU free to issue other waves'
+// instructions while they run.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned pk_bf16(float lo, float hi) {
+  const bf16x2 v = {(__bf16)lo, (__bf16)hi};
+  return __builtin_bit_cast(unsigned, v);
+}
+__device__ __forceinline__ float bf_lo(unsigned u) { return __uint_as_float(u << 16); }
+__device__ __forceinline__ float bf_hi(unsigned u) { return __uint_as_float(u & 0xffff0000u); }
+
+struct Split3 {
+  unsigned h, m, l;  // [re part | im part << 16] at each level
+};
+__device__ __forceinline__ Split3 split3(float x, float y) {
+  Split3 s;
+  s.h = pk_bf16(x, y);
+  x -= bf_lo(s.h);
+  y -= bf_hi(s.h);
+  s.m = pk_bf16(x, y);
+  x -= bf_lo(s.m);
+  y -= bf_hi(s.m);
+  s.l = pk_bf16(x, y);
+  return s;
+}
+
+// the B operand of a value (br, bi): panel row k = g, one column
+__device__ __forceinline__ u32x4 bf_pack_b(float br, float bi) {
+  const Split3 s = split3(br, bi);
+  return (u32x4){s.h, s.m, s.l, s.h};
+}
+
+// the A operand of one lane (matrix row n, panel column k = g), both planes
+struct BfA {
+  u32x4 r1, r2, i1, i2;
+};
+__device__ __forceinline__ BfA bf_prep_a(float ar, float ai) {
+  const Split3 s = split3(ar, ai);
+  auto neg = [](unsigned u) { return u ^ 0x80000000u; };                  // [re | -im]
+  auto swp = [](unsigned u) { return __builtin_amdgcn_alignbit(u, u, 16); };  // [im | re]
+  BfA a;
+  a.r1 = (u32x4){neg(s.h), neg(s.h), neg(s.h), neg(s.m)};
+  a.r2 = (u32x4){neg(s.l), neg(s.m), neg(s.m), 0u};
+  a.i1 = (u32x4){swp(s.h), swp(s.h), swp(s.h), swp(s.m)};
+  a.i2 = (u32x4){swp(s.l), swp(s.m), swp(s.m), 0u};
+  return a;
+}
+// (cr, ci) += a b, b the packed B operand of this lane (bf_pack_b)
+__device__ __forceinline__ void bf_cmma(const BfA &a, u32x4 bp, mfma_traits<float>::v4 &cr,
+                                        mfma_traits<float>::v4 &ci) {
+  const bf16x8 b = __builtin_bit_cast(bf16x8, bp);
+  cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a.r1), b, cr, 0, 0, 0);
+  ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a.i1), b, ci, 0, 0, 0);
+  cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a.r2), b, cr, 0, 0, 0);
+  ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a.i2), b, ci, 0, 0, 0);
+}
+
 template <typename T>
 __device__ __forceinline__ typename mfma_traits<T>::c2 mkc(T x, T y) {
   typename mfma_traits<T>::c2 r;
