@@ -12,7 +12,8 @@ error against the FP64 path on Σ, χ and γ (the 1e-10 gate on Σ).
 
 Prints ONE JSON line.  The roofline entry is for the dominant kernel (the BSE),
 timed with HIP events on the stream it runs on; the CPU baseline times the
-oracle (numpy/scipy FP64, one thread) on a bounded sample of the same work.
+oracle (numpy/scipy FP64) on a bounded sample of the same work, on all the
+host cores available (at most 16) and on one.
 """
 from __future__ import annotations
 
@@ -49,7 +50,7 @@ def parse():
                     help="the headline path; the other one is timed beside it")
     ap.add_argument("--numax", type=int, default=-1, help="EoM νmax (default: the λdm window min(n_iν, 3Nω/8))")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-points", type=int, default=300, help="(q,ω) points in the CPU BSE sample")
+    ap.add_argument("--cpu-points", type=int, default=960, help="(q,ω) points in the CPU BSE sample")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (calibrates the HBM byte counters)")
     return ap.parse_args()
@@ -64,50 +65,139 @@ def copy_calibration(mib):
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_calib_bytes_read": n * 8, "copy_calib_bytes_written": n * 8}))
+    print(json.dumps({"copy_calib_bytes_read": n * 8, "copy_calib_bytes_written": n * 8}), file=_STDOUT)
 
 
-def cpu_baseline(inp, numax, npoints_sample):
-    """Oracle (numpy/scipy FP64, one thread) on a bounded sample of the step.
-
-    BSE: zgetrf+zgetri on `npoints_sample` random (q,ω) points, both channels.
-    Bubble and EoM: the full oracle passes, divided by the number of points.
-    """
+def _cpu_share(conn, barrier, spec):
+    """One process of the CPU baseline: its share of each stage (items i with i % nsplit == rank),
+    each repetition timed between barriers shared by all the processes of the run."""
     import numpy as np
+    from multiprocessing import shared_memory
     from threadpoolctl import threadpool_limits
     from oracle import ldga_oracle as O
 
-    cfg = inp.cfg
-    oK = O.KGrid(cfg.D, cfg.Ns, cfg.t, wedge_coords=inp.kG.wedge)
-    omP = O.ModelParameters(inp.mP.U, inp.mP.μ, inp.mP.β, inp.mP.n, 0, 0, inp.mP.Ekin_DMFT)
-    osP = O.SimulationParameters(cfg.n_iω, cfg.n_iν, 0, cfg.shift)
-    Nq, Nω = inp.kG.Nq, 2 * cfg.n_iω + 1
-    npts = Nq * Nω
-    with threadpool_limits(limits=1):
-        t0 = time.perf_counter()
-        χ0 = O.calc_bubble(inp.G_fft, inp.G_rfft, oK, omP, osP)
-        t_bub = time.perf_counter() - t0
-        rng = np.random.default_rng(0)
-        qs, ws = rng.integers(0, Nq, npoints_sample), rng.integers(0, Nω, npoints_sample)
-        t0 = time.perf_counter()
-        for q, w in zip(qs, ws):
-            for Γ in (inp.Γ_m, inp.Γ_d):
-                A = np.asfortranarray(Γ[:, :, w] + np.diag(1.0 / χ0[q, :, w]))
+    shms, arr = [], {}
+    for key, (name, shape, dt) in spec["arrays"].items():
+        m = shared_memory.SharedMemory(name=name)  # the parent owns and unlinks it
+        shms.append(m)
+        arr[key] = np.ndarray(shape, dtype=dt, buffer=m.buf, order="F")
+    D, Ns, t, wedge, U, mu, beta, n, ekin, n_iw, n_iν, shift, numax = spec["params"]
+    oK = O.KGrid(D, Ns, t, wedge_coords=wedge)
+    omP = O.ModelParameters(U, mu, beta, n, 0, 0, ekin)
+    osP = O.SimulationParameters(n_iw, n_iν, 0, shift)
+    rank, nsplit, reps = spec["rank"], spec["nsplit"], spec["reps"]
+    Nq, Nν, Nω = arr["chi0"].shape
+    pts = spec["points"][rank::nsplit]
+    ωs = set(range(rank, Nω, nsplit))
+    ones = np.broadcast_to(np.complex128(1.0), (Nq, Nν, Nω))
+    cm = np.full((Nq, Nω), 0.1)
+    times = {"bse": [], "bubble": [], "eom": []}
+
+    def bse():
+        for q, w in pts:
+            for Γ in (arr["Gm"], arr["Gd"]):
+                A = np.asfortranarray(Γ[:, :, w] + np.diag(1.0 / arr["chi0"][q, :, w]))
                 inv, _, _ = O.inv_lapack(A)
                 _ = np.real(inv.sum()), inv.sum(axis=0)
-        t_bse = (time.perf_counter() - t0) / npoints_sample
-        c2 = omP.Ekin_DMFT
-        γ = np.asfortranarray(np.ones((Nq, 2 * cfg.n_iν, Nω), dtype=complex))
-        cm = np.asfortranarray(np.full((Nq, Nω), 0.1))
-        t0 = time.perf_counter()
-        O.calc_Σ(O.ChiT(cm, omP.beta, tail_c=[0, 0, c2]), γ, O.ChiT(cm, omP.beta, tail_c=[0, 0, c2]), γ, 0.0, γ,
-                 inp.G_rfft, oK, omP, osP, νmax=numax)
-        t_eom = time.perf_counter() - t0
-    per_point = t_bse + (t_bub + t_eom) / npts
-    return {"value": 1.0 / per_point, "unit": "(q,ω)-points/s", "cores": 1, "kind": "port",
-            "sample": f"oracle on the {cfg.Ns}^{cfg.D} workload, 1 thread: zgetrf+zgetri (both channels) on "
-                      f"{npoints_sample} random (q,ω) points ({t_bse * 1e3:.2f} ms/point) + full FFT bubble "
-                      f"({t_bub:.2f} s) + full FFT EoM at νmax={numax} ({t_eom:.2f} s), per point over {npts}"}
+
+    stages = {"bse": bse,
+              "bubble": lambda: O.calc_bubble(arr["gf"], arr["gr"], oK, omP, osP, ωsel=ωs),
+              "eom": lambda: O.calc_Σ_ω(O.EOM["full"], cm, ones, cm, ones, arr["gr"], ones, U, oK, osP, numax,
+                                        ωsel=ωs)}
+    with threadpool_limits(limits=1):
+        for k, f in stages.items():
+            for _ in range(reps):
+                barrier.wait()
+                t0 = time.perf_counter()
+                f()
+                times[k].append(time.perf_counter() - t0)
+    conn.send(times)
+    conn.close()
+    del arr
+    for m in shms:
+        m.close()
+
+
+def _cpu_run(spec0, nproc, nsplit):
+    """Run the CPU baseline's stages on nproc processes (ranks 0..nproc-1 of an nsplit-way split);
+    per stage and repetition the wall time is the slowest process's."""
+    import multiprocessing as mp
+    import numpy as np
+    ctx = mp.get_context("spawn")
+    barrier = ctx.Barrier(nproc, timeout=300)
+    procs, pipes = [], []
+    for r in range(nproc):
+        a, b = ctx.Pipe(duplex=False)
+        p = ctx.Process(target=_cpu_share, args=(b, barrier, dict(spec0, rank=r, nsplit=nsplit)), daemon=True)
+        p.start()
+        procs.append(p)
+        pipes.append(a)
+    res = [a.recv() for a in pipes]
+    for p in procs:
+        p.join(60)
+    return {k: float(np.median(np.max([r[k] for r in res], axis=0))) for k in res[0]}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(inp, numax, npoints_sample, χ0, reps=5):
+    """Oracle (numpy/scipy FP64) on a bounded sample of the step, on all the host cores this
+    process may use (at most 16) and on one core.
+
+    BSE: zgetrf+zgetri on `npoints_sample` random (q,ω) points, both channels.  Bubble and EoM: the
+    full oracle passes (their ω loop split over the processes), divided by the number of points.
+    One process per core (the LAPACK and FFT calls hold the interpreter lock), one BLAS thread each;
+    per stage the median over `reps` repetitions of the slowest process.  The one-core value runs
+    one process on one share of the same split and scales it by the number of shares.
+    """
+    import numpy as np
+    from multiprocessing import shared_memory
+
+    cfg = inp.cfg
+    Nq, Nω = inp.kG.Nq, 2 * cfg.n_iω + 1
+    npts = Nq * Nω
+    cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+    rng = np.random.default_rng(0)
+    points = list(zip(rng.integers(0, Nq, npoints_sample).tolist(), rng.integers(0, Nω, npoints_sample).tolist()))
+    src = {"gf": inp.G_fft, "gr": inp.G_rfft, "Gm": inp.Γ_m, "Gd": inp.Γ_d, "chi0": χ0}
+    shms, arrays = [], {}
+    try:
+        for k, a in src.items():
+            a = np.asfortranarray(a)
+            m = shared_memory.SharedMemory(create=True, size=max(a.nbytes, 1))
+            shms.append(m)
+            np.ndarray(a.shape, dtype=a.dtype, buffer=m.buf, order="F")[...] = a
+            arrays[k] = (m.name, a.shape, a.dtype.str)
+        mp_ = inp.mP
+        spec0 = {"arrays": arrays, "points": points, "reps": reps,
+                 "params": (cfg.D, cfg.Ns, cfg.t, inp.kG.wedge, mp_.U, mp_.μ, mp_.β, mp_.n, mp_.Ekin_DMFT,
+                            cfg.n_iω, cfg.n_iν, cfg.shift, numax)}
+        t_all = _cpu_run(spec0, cores, cores)
+        t_one = {k: v * cores for k, v in _cpu_run(spec0, 1, cores).items()}
+    finally:
+        for m in shms:
+            m.close()
+            m.unlink()
+
+    def rate(t):
+        return 1.0 / (t["bse"] / npoints_sample + (t["bubble"] + t["eom"]) / npts)
+
+    per_stage = {f"{k}_{lab}_s": round(t[k], 4) for lab, t in (("all_cores", t_all), ("one_core", t_one)) for k in t}
+    return {"value": rate(t_all), "unit": "(q,ω)-points/s", "cores": cores, "kind": "port",
+            "value_one_core": rate(t_one), "cpu_model": cpu_model(), "repetitions": reps,
+            "stages": per_stage,
+            "sample": f"oracle on the {cfg.Ns}^{cfg.D} workload, {cores} processes x 1 BLAS thread: zgetrf+zgetri "
+                      f"(both channels) on {npoints_sample} random (q,ω) points + the full FFT bubble + the full FFT "
+                      f"EoM at νmax={numax} (ω loops split over the processes), per point over {npts}; median of "
+                      f"{reps}; one-core value from one process on 1/{cores} of the same work"}
 
 
 def lambdadm_step_costs(L, ctx, o, λ0, inp, numax, Nq, Nν, Nω, reps=5):
@@ -140,6 +230,53 @@ def lambdadm_step_costs(L, ctx, o, λ0, inp, numax, Nq, Nν, Nω, reps=5):
     return {"lambda": lam, "residual_ms": t_res * 1e3, "residual_and_adjoint_jacobian_ms": t_jac * 1e3,
             "fd_newton_step_ms": 3 * t_res * 1e3, "adjoint_newton_step_ms": t_jac * 1e3,
             "step_speedup": 3 * t_res / t_jac, "mu_iters": int(r["mu_iters"])}
+
+
+def lambdadm_convergence(L, ctx, o, λ0, inp, numax, Nq, Nν, Nω):
+    """λdm_correction (conditions_new.jl:207-324) solved to convergence from λ = (0, 0) on this step's
+    ladder arrays, once with the reference's forward finite-difference Jacobian (three residuals per
+    Newton step) and once with the adjoint Jacobian; wall time of the whole call (including the one
+    upload of χ, γ, λ₀), Newton steps, residual evaluations and whether the root was found."""
+    import numpy as np
+    Fo = lambda a, shape: np.asfortranarray(a.cpu().numpy().reshape(shape, order="F"))
+    χm = L.χT(Fo(o["χm"], (Nq, Nω)), inp.mP.β, tail_c=[0, 0, inp.mP.Ekin_DMFT])
+    χd = L.χT(Fo(o["χd"], (Nq, Nω)), inp.mP.β, tail_c=[0, 0, inp.mP.Ekin_DMFT])
+    γm, γd = L.γT(Fo(o["γm"], (Nq, Nν, Nω))), L.γT(Fo(o["γd"], (Nq, Nν, Nω)))
+    l0 = Fo(λ0, (Nq, Nν, Nω))
+    out = {}
+    for jac in ("fd", "adjoint"):
+        t0 = time.perf_counter()
+        r = L.λdm_correction(χm, γm, χd, γd, inp.Σ_loc, inp.G_rfft, inp.χloc_m_sum, l0, inp.kG, inp.mP, inp.sP,
+                             νmax=numax, jacobian=jac, ctx=ctx)
+        out[jac] = {"wall_ms": (time.perf_counter() - t0) * 1e3, "converged": bool(r.converged),
+                    "newton_steps": int(r.newton_steps), "residual_evals": int(r.residual_evals),
+                    "lambda": [float(r.λm), float(r.λd)]}
+    return out
+
+
+def lambdadm_convergence_rooted(L, ctx):
+    """The same on a 2D 64x64 system whose λdm conditions have a root (U=2, β=10, n=0.9, Nν=60, Nω=61,
+    vertex noise 0.1): its chain on the GPU in FP64, then both Newton variants to convergence."""
+    import warnings
+    from ladderdga_hip import synthetic
+    cfg = synthetic.tiny_config(D=2, Ns=64, n_iν=30, n_iω=30, n=0.9, U=2.0, β=10.0, noise=0.1)
+    inp = synthetic.build_inputs(cfg)
+    χ0 = L.calc_bubble(inp.G_fft, inp.G_rfft, inp.kG, inp.mP, inp.sP, ctx=ctx)
+    ch = {}
+    for c, Γ in (("m", inp.Γ_m), ("d", inp.Γ_d)):
+        ch["χ" + c], ch["γ" + c] = L.calc_χγ(c, Γ, χ0, inp.kG, inp.mP, inp.sP, precision="fp64", ctx=ctx)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        λ0 = L.calc_λ0(χ0, inp.F_m, inp.mP, inp.sP, inp.kG, ctx=ctx)
+    out = {"workload": "2D square 64x64 U=2 β=10 n=0.9 Nν=60 Nω=61 (vertex noise 0.1)"}
+    for jac in ("fd", "adjoint"):
+        t0 = time.perf_counter()
+        r = L.λdm_correction(ch["χm"], ch["γm"], ch["χd"], ch["γd"], inp.Σ_loc, inp.G_rfft, inp.χloc_m_sum, λ0,
+                             inp.kG, inp.mP, inp.sP, jacobian=jac, ctx=ctx)
+        out[jac] = {"wall_ms": (time.perf_counter() - t0) * 1e3, "converged": bool(r.converged),
+                    "newton_steps": int(r.newton_steps), "residual_evals": int(r.residual_evals),
+                    "lambda": [float(r.λm), float(r.λd)]}
+    return out
 
 
 def run_sc_iteration_cost(L, ctx, o, λ0, inp, numax, Nq, Nν, Nω):
@@ -320,6 +457,9 @@ def main():
                        "max_rel_resid": rp.max_rel_resid}
     result["mixed_bse_report"] = bse_rep
     result["lambdadm_newton_step"] = lambdadm_step_costs(L, ctx, out[main_prec], λ0, inp, numax, Nq, Nν, Nω)
+    result["lambdadm_convergence"] = {"workload": cfg.name, **lambdadm_convergence(L, ctx, out[main_prec], λ0, inp,
+                                                                                    numax, Nq, Nν, Nω)}
+    result["lambdadm_convergence_rooted"] = lambdadm_convergence_rooted(L, ctx)
     result["run_sc_iteration"] = run_sc_iteration_cost(L, ctx, out[main_prec], λ0, inp, numax, Nq, Nν, Nω)
     # G tables (helpers.jl:153-183) from the wedge G_loc, as the self-consistency loop rebuilds them:
     # algorithmic bytes = wedge G read + both full tables written
@@ -339,10 +479,16 @@ def main():
     result["g_tables"] = {"ms": gt_ms, "GBps": gt_bytes / (gt_ms * 1e-3) / 1e9,
                           "max_rel_err_vs_input_tables": max(rel(ga, Gf), rel(gb, Gr))}
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(inp, numax, args.cpu_points)
+        χ0h = χ0.cpu().numpy().reshape((Nq, Nν, Nω), order="F")
+        result["cpu_baseline"] = cpu_baseline(inp, numax, args.cpu_points, χ0h)
     ctx.close()
-    print(json.dumps(result, ensure_ascii=False))
+    print(json.dumps(result, ensure_ascii=False), file=_STDOUT)
 
+
+# the JSON line is the only output on stdout: everything else the run prints goes to stderr
+_STDOUT = sys.stdout
 
 if __name__ == "__main__":
-    main()
+    import contextlib
+    with contextlib.redirect_stdout(sys.stderr):
+        main()

@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import sys
 import warnings
 
 import numpy as np
@@ -187,7 +188,7 @@ def χ_λ_inplace(χ: χT, λ: float):
     if λ == 0.0:
         return
     if not math.isfinite(λ):
-        print(f"WARNING. SKIPPING λ correction because {λ} is not finite!")
+        print(f"WARNING. SKIPPING λ correction because {λ} is not finite!", file=sys.stderr)
         return
     χ.data[...] = χ.data / (λ * χ.data + 1)
     χ.λ = χ.λ + λ

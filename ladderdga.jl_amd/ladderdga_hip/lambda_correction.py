@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import sys
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -103,12 +104,12 @@ def newton_right(f, start, min_, nsteps: int = 500, atol: float = 1e-8, max_rese
         xi_last = xi.copy()
         xi = xi - dfii @ fi
         if verbose:
-            print(f"i={i} : xi = {xi}, fi = {fi}. {np.linalg.norm(fi)} ?<? {atol}")
+            print(f"i={i} : xi = {xi}, fi = {fi}. {np.linalg.norm(fi)} ?<? {atol}", file=sys.stderr)
         for l in range(len(xi)):
             if xi[l] < min_[l]:
                 xi[l] = min_[l] + abs(xi_last[l] - min_[l]) / 2
                 if abs(xi_last[l] - min_[l]) / 2 < 1e-8:
-                    print("WARNING: Bisection step is smaller than 1e-8. This indicates faulty data or no solution.")
+                    print("WARNING: Bisection step is smaller than 1e-8. This indicates faulty data or no solution.", file=sys.stderr)
                     bisection_fail = True
         if bisection_fail or not np.all(np.isfinite(fi)):
             bisection_fail = False
@@ -263,12 +264,12 @@ def Î»dm_correction(Ï‡m: Ï‡T, Î³m: Î³T, Ï‡d: Ï‡T, Î³d: Î³T, Î£_loc, gLoc_rfft, Ï
         root, steps = newton_right(f, list(Î»init), min_Î», verbose=verbose,
                                    fj=fj if jacobian == "adjoint" else None)
     except RuntimeError as e:
-        print(f"Error: {e}")
+        print(f"Error: {e}", file=sys.stderr)
         root, steps = np.array([math.nan, math.nan]), 0
     if np.any(np.isnan(root)):
-        print("WARNING: No Î» root was found!")
+        print("WARNING: No Î» root was found!", file=sys.stderr)
     elif np.any(root < min_Î»):
-        print(f"WARNING: Î» = {root} outside region ({min_Î»})!")
+        print(f"WARNING: Î» = {root} outside region ({min_Î»})!", file=sys.stderr)
     if np.all(np.isfinite(root)):
         r = prob.eval(root, fit_Î¼=fit_Î¼, arrays=True)
         converged = abs(rhs_c1 - r["lhs_c1"]) <= validate_threshold and abs(r["Epot1"] - r["Epot2"]) <= validate_threshold
@@ -370,7 +371,7 @@ def run_sc(Ï‡m: Ï‡T, Î³m: Î³T, Ï‡d: Ï‡T, Î³d: Î³T, Î»0, Î¼: float, Î£_loc, gLoc_
                 if math.isfinite(r.Î¼):
                     Î¼ = r.Î¼
             if not (math.isfinite(Î»m) and math.isfinite(Î»d)) or not validation:
-                print(f"ERROR: Î»m = {Î»m} or Î»d = {Î»d} not finite, OR internal Î» correction did not find root!")
+                print(f"ERROR: Î»m = {Î»m} or Î»d = {Î»d} not finite, OR internal Î» correction did not find root!", file=sys.stderr)
                 break
             ctx.check(L.ldga_sigma_dev(ctx.handle, C.byref(fr), C.byref(mo), Î½max, Î»m, Î»d, int(tc), 0,
                                        float(Ï‡loc_m_sum), d["Ï‡m"].ptr, d["Î³m"].ptr, d["Ï‡d"].ptr, d["Î³d"].ptr,

@@ -284,13 +284,17 @@ def G_fft_from_wedge(G_wedge, kG: KGrid):
 # ----------------------------------------------------------------------------
 
 
-def calc_bubble(g_fft, g_rfft, kG: KGrid, mP, sP):
-    """chi0[q, nu, omega] = -beta conv(G(nu), G(nu+omega)).  Local case Nk=1 is a product."""
+def calc_bubble(g_fft, g_rfft, kG: KGrid, mP, sP, ωsel=None):
+    """chi0[q, nu, omega] = -beta conv(G(nu), G(nu+omega)).  Local case Nk=1 is a product.
+    ωsel (optional): only these omega indices are computed, the others stay 0 -- the same loop
+    on a subset, for splitting the pass over processes."""
     f0 = sP.fft_range.start
     Nν = 2 * (sP.n_inu + sP.n_inu_shell)
     Nω = 2 * sP.n_iw + 1
-    data = np.empty((kG.Nq, Nν, Nω), dtype=complex)
+    data = np.empty((kG.Nq, Nν, Nω), dtype=complex) if ωsel is None else np.zeros((kG.Nq, Nν, Nω), dtype=complex)
     for ωi, ωn in enumerate(range(-sP.n_iw, sP.n_iw + 1)):
+        if ωsel is not None and ωi not in ωsel:
+            continue
         νrange = [v - trunc_div2(sP.shift * ωn) for v in
                   range(-(sP.n_inu + sP.n_inu_shell), sP.n_inu + sP.n_inu_shell)]
         for νi, νn in enumerate(νrange):
@@ -523,14 +527,17 @@ def conv_eom(kG: KGrid, K_wedge, g_rfft_slice):
     return conv_fft(kG, Kf, g_rfft_slice)
 
 
-def calc_Σ_ω(eomf, χm, γm, χd, γd, g_rfft, λ0, U, kG, sP, νmax, per_ω=False, νsel=None):
+def calc_Σ_ω(eomf, χm, γm, χd, γd, g_rfft, λ0, U, kG, sP, νmax, per_ω=False, νsel=None, ωsel=None):
     """src/ladderDGATools_singleCore.jl:28-69: the omega/nu/q loop nest (chi already lambda-mapped).
-    νsel (optional): only these ν columns (0-based ν_n) are summed, the others stay 0 -- the same
-    loop on a subset, for grids where the whole nest takes too long."""
+    νsel / ωsel (optional): only these ν columns (0-based ν_n) / omega indices are summed, the
+    others stay 0 -- the same loop on a subset, for grids where the whole nest takes too long
+    and for splitting it over processes."""
     Nq, Nω = χm.shape
     f0 = sP.fft_range.start
     Σ = np.zeros((Nq, νmax, Nω) if per_ω else (Nq, νmax), dtype=complex)
     for ωi in range(Nω):
+        if ωsel is not None and ωi not in ωsel:
+            continue
         ωn = ωi - sP.n_iw
         νZero = ν0Index_of_ωIndex(ωi + 1, sP)
         νlist = list(range(νZero, 2 * sP.n_inu + 1))[:νmax]
@@ -669,6 +676,26 @@ def secant(f, x0, tol=1e-13, maxit=100):
                 fb *= 0.5
             side = 1
     return c
+
+
+def calc_E_ED(ϵk, Vk, GImp, U, n, μ, β):
+    """src/thermodynamics.jl:46-64: kinetic and potential energy of the Anderson impurity from its
+    bath (ϵk, Vk) and G_imp(iν_n), n = 0..len-1, with the 1/(iν)^2 tails summed analytically."""
+    ϵk, Vk, GImp = np.asarray(ϵk, float), np.asarray(Vk, float), np.asarray(GImp, complex)
+    vk = float(np.sum(Vk ** 2))
+    Σ_h = n * U / 2
+    E_pot_tail = (U ** 2) / 2 * n * (1 - n / 2) - Σ_h * (Σ_h - μ)
+    E_kin_tail = vk
+    iν = iν_array(β, range(len(GImp)))
+    E_kin = E_pot = 0.0
+    for m in range(len(GImp)):
+        Δ = np.sum((Vk * np.conj(Vk)) / (iν[m] - ϵk))
+        Σ_n = iν[m] - Δ - 1.0 / GImp[m] + μ
+        E_kin += 2 * np.real(GImp[m] * Δ - E_kin_tail / iν[m] ** 2)
+        E_pot += 2 * np.real(GImp[m] * Σ_n - E_pot_tail / iν[m] ** 2)
+    E_kin = E_kin * (2 / β) - (β / 2) * E_kin_tail
+    E_pot = E_pot * (1 / β) + 0.5 * Σ_h - (β / 4) * E_pot_tail
+    return float(E_kin), float(E_pot)
 
 
 def calc_E(G, Σ, μ, kG, mP, sP):

@@ -338,3 +338,34 @@ def test_G_from_Σladder(ctx, fix_n, μ):
     if fix_n:
         f0 = osP.fft_range.start
         assert O.filling_pos_kG(G[:, -f0:-f0 + Σ.shape[1]], oK, omP.U, μg, omP.beta) == pytest.approx(omP.n, abs=1e-10)
+
+
+def test_lambdadm_convergence_fd_vs_adjoint(ctx):
+    """λdm solved to convergence from (0, 0) on a 16x16 system with a root (U = 1.4, β = 4, n = 0.9,
+    Nν = 32, Nω = 33, no vertex noise): the reference's Newton with its forward-difference Jacobian and
+    the adjoint Newton reach the same root, a root of the oracle's residual (this system has more than
+    one: the oracle's own Newton, whose μ fits all restart from mP.μ, walks to another one); the adjoint
+    takes one residual evaluation per step instead of three.  Wall times are printed."""
+    import time
+    import ladderdga_hip as L
+    from ladderdga_hip import synthetic
+    cfg = synthetic.tiny_config(D=2, Ns=16, n_iν=16, n_iω=16, n=0.9, U=1.4, β=4.0, noise=0.0)
+    TINY["conv16"] = dict(D=2, Ns=16, n_iν=16, n_iω=16, n=0.9, U=1.4, β=4.0, noise=0.0)
+    inp, oK, omP, osP, o, p, νmax = ladder("conv16")
+    assert inp.cfg == cfg
+    res = {}
+    for jac in ("fd", "adjoint"):
+        t0 = time.perf_counter()
+        r = L.λdm_correction(p["χm"], p["γm"], p["χd"], p["γd"], inp.Σ_loc, inp.G_rfft, inp.χloc_m_sum, o["λ0"],
+                             inp.kG, inp.mP, inp.sP, jacobian=jac, ctx=ctx)
+        dt = time.perf_counter() - t0
+        print(f"λdm {jac}: {dt * 1e3:.1f} ms, {r.newton_steps} steps, {r.residual_evals} residuals, "
+              f"root ({r.λm}, {r.λd})", flush=True)
+        assert r.converged
+        ref = O.λdm_residual_vals(np.array([r.λm, r.λd]), o["χm"], o["γm"], o["χd"], o["γd"], inp.Σ_loc, inp.G_rfft,
+                                  inp.χloc_m_sum, o["λ0"], oK, omP, osP, νmax, r.μ)
+        assert abs(ref[5] - omP.n / 2 * (1 - omP.n / 2)) <= 1e-8 and abs(ref[3] - ref[4]) <= 1e-8
+        assert ref[1] == pytest.approx(r.μ, abs=1e-10)
+        res[jac] = r
+    assert np.allclose([res["fd"].λm, res["fd"].λd], [res["adjoint"].λm, res["adjoint"].λd], rtol=1e-6, atol=1e-6)
+    assert res["adjoint"].residual_evals < res["fd"].residual_evals

@@ -413,3 +413,29 @@ def test_sum_ω_strict_usable_range_known_answer():
         wl, rel=1e-14)
     with pytest.raises(ValueError):
         O.sum_ω(O.ChiT(d, β, tail_c=[0.0, 1.0, 1.0]))
+
+
+def test_omega_subsets_partition_bubble_and_eom():
+    """The ω-subset options (used to split the CPU baseline over processes) partition the passes:
+    the subsets of a 3-way split add up to the full bubble and the full EoM sum, bit for bit
+    per ω slice (each slice is computed by the same calls)."""
+    kG, sP_, mP, Σ_loc, gLoc, gf, gr, rng = _tiny_inputs(Ns=4, n_inu=3, n_iw=2)
+    Nq, Nν, Nω = kG.Nq, 2 * sP_.n_inu, 2 * sP_.n_iw + 1
+    full = O.calc_bubble(gf, gr, kG, mP, sP_)
+    parts = [O.calc_bubble(gf, gr, kG, mP, sP_, ωsel=set(range(r, Nω, 3))) for r in range(3)]
+    for r, p in enumerate(parts):
+        sel = list(range(r, Nω, 3))
+        assert np.array_equal(p[:, :, sel], full[:, :, sel])
+        assert not np.any(np.delete(p, sel, axis=2))
+    cm = rng.uniform(0.1, 0.5, (Nq, Nω))
+    gm = rng.standard_normal((Nq, Nν, Nω)) + 1j * rng.standard_normal((Nq, Nν, Nω))
+    l0 = rng.standard_normal((Nq, Nν, Nω)) + 1j * rng.standard_normal((Nq, Nν, Nω))
+    args = (O.EOM["full"], cm, gm, cm, gm, gr, l0, mP.U, kG, sP_, 3)
+    per = O.calc_Σ_ω(*args, per_ω=True)
+    for r in range(3):
+        sel = list(range(r, Nω, 3))
+        p = O.calc_Σ_ω(*args, per_ω=True, ωsel=set(sel))
+        assert np.array_equal(p[:, :, sel], per[:, :, sel])
+        assert not np.any(np.delete(p, sel, axis=2))
+    tot = sum(O.calc_Σ_ω(*args, ωsel=set(range(r, Nω, 3))) for r in range(3))
+    assert np.allclose(tot, O.calc_Σ_ω(*args), rtol=1e-14, atol=1e-14)
