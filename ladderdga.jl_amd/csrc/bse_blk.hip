@@ -49,6 +49,11 @@ namespace {
 
 using namespace ldga_bse;
 
+// smallest FP32 tile count that runs the rank-4 updates on bf16 MFMA: below it
+// the workgroups are few per CU and the factoring chain, which the bf16 form
+// lengthens by packing the B operand, is the critical path
+constexpr int kBfMinNT = 6;
+
 template <typename T, int NT>
 struct BlkShared {
   typedef typename mfma_traits<T>::c2 c2;
@@ -56,13 +61,17 @@ struct BlkShared {
   // rows padded so that rows 1024 bytes apart (NC = 128) do not fall on the
   // same LDS banks (the column reads of S and the per-group reads of W)
   static constexpr int SPAD = sizeof(T) == 8 ? 4 : 2, WPAD = sizeof(T) == 8 ? 32 : 16;
-  static constexpr bool BF = std::is_same<T, float>::value;
+  // FP32 from NT = kBfMinNT: the rank-4 updates run on bf16 MFMA with the split
+  // B operand published in Wb (BF), and to make room for it only the panel's
+  // four rows are staged (S4); otherwise the owner stages all sixteen of its rows
+  static constexpr bool BF = std::is_same<T, float>::value && NT >= kBfMinNT;
+  static constexpr bool S4 = BF;
   c2 W[3][4][NC + WPAD];   // published panel update (triple-buffered)
   // FP32: W as the bf16 MFMA B operand (bf_pack_b), published with it
   u32x4 Wb[BF ? 3 : 1][4][BF ? NC + 1 : 1];
   // staged by a panel's owner (triple-buffered): FP32 the panel's four rows
   // (row r: lane group r), FP64 all sixteen rows of the owner
-  c2 S[3][BF ? 4 : 16][NC + SPAD];
+  c2 S[3][S4 ? 4 : 16][NC + SPAD];
   T Cr[NT][16][4];     // per wave: own rows at the four pivot columns (real)
   T Ci[NT][16][4];     // (imaginary)
   int P[3][4];         // pivot columns of the panel
@@ -210,7 +219,7 @@ __device__ __forceinline__ void blk_solve(BlkShared<T, NT> &sh, const int lb, in
   // group r), FP64 all sixteen (tile row order)
   auto stage_rows = [&](int m) {
     const int sb = m % 3;
-    if constexpr (!std::is_same<T, float>::value) {
+    if constexpr (!BlkShared<T, NT>::S4) {
 #pragma unroll
       for (int t = 0; t < NT; ++t)
 #pragma unroll
@@ -245,7 +254,7 @@ __device__ __forceinline__ void blk_solve(BlkShared<T, NT> &sh, const int lb, in
   // tile t += A W[wb] (a: tile row n, k = g).  FP64: four K=4 f64 MFMAs.
   // FP32: the A operand split once per panel (prep), the B operand as
   // published (Wb), four bf16 MFMAs per tile (bf_cmma).
-  constexpr bool kBf = std::is_same<T, float>::value;
+  constexpr bool kBf = BlkShared<T, NT>::BF, kS4 = BlkShared<T, NT>::S4;
   typedef typename std::conditional<kBf, BfA, c2>::type AOp;
   auto prep = [&](c2 a) -> AOp {
     if constexpr (kBf) return bf_prep_a(a.x, a.y);
@@ -267,14 +276,14 @@ __device__ __forceinline__ void blk_solve(BlkShared<T, NT> &sh, const int lb, in
   };
   // the owner of panel kp, after factoring it: its tile += A1 W[w1] + A2 W[w2]
   // for panels kp-1 (pivots p, deferred) and kp (pivots q).  Its rows are as
-  // of panel kp-2 (it factored panel kp meanwhile): in its accumulators, and
-  // for FP64 also staged in S[sb].  A1 = M[:, P], A2 = M[:, Q] + A1 W[w1][:, Q].
+  // of panel kp-2 (it factored panel kp meanwhile): in its accumulators, and,
+  // when all sixteen are staged, in S[sb].  A1 = M[:, P], A2 = M[:, Q] + A1 W[w1][:, Q].
   // (The panel's own rows of the tile are overwritten with their final values
   // afterwards.)
   auto update_tile2 = [&](int sb, int p0, int p1, int p2, int p3, int w1, int q0, int q1, int q2, int q3, int w2) {
     const int qg = Lg == 0 ? q0 : Lg == 1 ? q1 : Lg == 2 ? q2 : q3;
     c2 a1, a2, cp[4];
-    if constexpr (kBf) {  // the FP32 accumulators are read by VGPR indexing
+    if constexpr (kS4) {  // the FP32 accumulators are read by VGPR indexing
       gather_cols(p0, p1, p2, p3);
       wave_sync_lds();
       a1 = mkc<T>(sh.Cr[wave][Ln][Lg], sh.Ci[wave][Ln][Lg]);
@@ -323,7 +332,7 @@ __device__ __forceinline__ void blk_solve(BlkShared<T, NT> &sh, const int lb, in
     const int sb = kp1 % 3;
     if constexpr (!pre) wave_sync_lds();  // own staging just before (later: a barrier in between)
     // the panel's rows in the staged copy (row r: element j1 of lane group r)
-    auto prow = [&](int r) { return kBf ? r : MT::trow(r, j1); };
+    auto prow = [&](int r) { return kS4 ? r : MT::trow(r, j1); };
     // the panel rows from the staged copy, one column per lane
     T pr[NU][4], pim[NU][4];
 #pragma unroll
