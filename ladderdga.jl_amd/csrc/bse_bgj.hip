@@ -1,0 +1,520 @@
+// Batched Bethe-Salpeter solve, mixed precision: a 16-row blocked
+// Gauss-Jordan inversion on MFMA with FP64 iterative refinement
+// (calc_χγ, src/BSETools.jl:192-248; replaces the per-(q,w) zgetrf+zgetri of
+// src/LapackWrapper.jl:64-72).
+//
+// For every (q,w) one workgroup of NT waves inverts M = A^T,
+// A = Gamma(w) + diag(1/chi0(q,.,w)), padded to NC = 16 NT with identity rows
+// and columns.  Wave w holds rows 16w .. 16w+15 in MFMA accumulators as NT
+// tiles of 16x16 (real and imaginary planes; f32 16x16x4 layout: lane
+// (g = lane/16, n = lane%16) holds, in element j, row 4g+j of the tile at
+// column n).  Block K = the rows and columns 16K .. 16K+15.
+//
+// Sweep of block K (the in-place block Gauss-Jordan step, P = inv(M[K,K])):
+//   M[K,K] <- P,  M[K,J] <- -P M[K,J],  M[I,K] <- M[I,K] P,
+//   M[I,J] <- M[I,J] - M[I,K] P M[K,J]        (I, J != K),
+// written as  W = (new rows of K) - E_K,  M[I,:] += M[I,K] W  for I != K.
+// After the sweeps of all blocks M holds inv(M) in natural order, and the
+// solution of A^T y = 1 is y = inv(M) 1 (row sums).
+//
+// Pivoting is local to the diagonal block: D = M[K,K] is inverted by scalar
+// Gauss-Jordan steps with column pivoting among D's own unpivoted columns
+// (LAPACK's izamax criterion |re|+|im|, first maximum).  A D that is exactly
+// singular (zero pivot), or a refinement that does not converge, sends the
+// matrix to the FP64 re-solve with partial pivoting over the whole row; the
+// refinement below restores FP64 accuracy for every other matrix.
+//
+// Step k (owner = wave k, the owner of the block's rows):
+//   barrier 1: P = inv(D_k) and the owner's rows M[K,:] are in LDS;
+//   phase 1: wave w forms tile w of W: -P M[K, tile w] (four f32 K=4 MFMA
+//     chunks, complex 4M scheme), or P - I for w = K, and publishes it split
+//     for the bf16 MFMA B operand;
+//   barrier 2;
+//   phase 2: the owner takes W + E_K as its rows; every other wave applies
+//     M[I,:] += M[I,K] W as four rank-4 complex bf16 MFMA updates per tile
+//     (A operand: its tile K, transposed through LDS); the owner of step k+1
+//     (raised wave priority) then publishes its rows and inverts its diagonal
+//     tile, so that its chain (rank-16 update, sixteen pivot steps) is the
+//     critical path of the step.
+//
+// FP32 products on bf16 MFMA: an FP32 value is split into three bf16 parts
+// (bse_common.h, split3); seven of the nine part products are taken, so each
+// product is below one FP32 multiplication's error.
+//
+// Refinement: r = 1 - A^T y with Gamma and chi0 in FP64 (Gamma is L2-resident:
+// all q of one w share it), y += X r, until |X r|_inf <= 2^-50 |y|_inf, or the
+// next correction extrapolated from the contraction rate is <= 2^-48 |y|_inf,
+// or the correction stops halving (the FP64 residual's rounding level).
+#include <algorithm>
+#include <cstring>
+
+#include "ldga_internal.h"
+
+#include "bse_common.h"
+
+namespace {
+
+using namespace ldga_bse;
+
+template <int NT>
+struct BgjShared {
+  static constexpr int NC = 16 * NT;
+  float2 rows[16][NC + 1];  // the step owner's rows, as of the previous step
+  u32x4 Wb[4][NT][64];      // W split for the bf16 B operand: [chunk][tile][lane (k row = lane / 16)]
+  float2 P[16][17];         // inv(D) of the step's diagonal block, natural order
+  float2 T[NT][16][17];     // per wave: its tile K transposed into the A operand layout
+  double2 y[NC];            // solution (FP64)
+  double2 dinv[NC];         // 1/chi0 on the diagonal (FP64)
+  float2 res[NC];           // refinement residual (rounded to FP32 for X r)
+  double red[NT];
+  float redf[2][2][NT];
+  int zero_pivot;
+};
+
+// maximum over the 16 lanes of each DPP row, in every lane of the row
+__device__ __forceinline__ unsigned row_max16_u32(unsigned k) {
+  __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf" : "+v"(k));
+  __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf" : "+v"(k));
+  __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 row_half_mirror row_mask:0xf bank_mask:0xf" : "+v"(k));
+  __asm__("s_nop 1\n\tv_max_u32_dpp %0, %0, %0 row_mirror row_mask:0xf bank_mask:0xf" : "+v"(k));
+  return k;
+}
+
+// the value of v in lane src (any lane of the wave)
+__device__ __forceinline__ float bperm(int src, float v) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
+}
+
+// an FP32 value from its packed split (h + m + l, exact)
+__device__ __forceinline__ float unsplit_lo(u32x4 u) { return (bf_lo(u[0]) + bf_lo(u[1])) + bf_lo(u[2]); }
+__device__ __forceinline__ float unsplit_hi(u32x4 u) { return (bf_hi(u[0]) + bf_hi(u[1])) + bf_hi(u[2]); }
+
+// four waves per SIMD (<= 128 VGPRs) up to eight tiles: two workgroups share a
+// CU and overlap their serial chains
+template <int NT>
+constexpr int bgj_min_waves() { return NT <= 8 ? 4 : 1; }
+
+// One workgroup solves matrix lb = q + Nq w.
+template <int NT>
+__device__ __forceinline__ void bgj_solve(BgjShared<NT> &sh, const int lb, const int N, const int Nq, const int Nc,
+                                          const int shell, const double2 *__restrict__ Gam,
+                                          const double2 *__restrict__ chi0, const double s, const double U,
+                                          const double beta, const int max_ir, double *__restrict__ chi_out,
+                                          double2 *__restrict__ gamma_out, unsigned long long *__restrict__ counters,
+                                          int *__restrict__ redo) {
+  typedef mfma_traits<float> MT;
+  typedef MT::v4 v4;
+  constexpr int NC = 16 * NT;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), g = lane >> 4,
+            n = lane & 15;
+  const int q = lb % Nq, w = lb / Nq;
+  const double2 *G = Gam + (size_t)w * N * N;
+  const int NB = (N + 15) >> 4;  // blocks holding matrix rows; the rest are identity blocks
+
+  for (int i = tid; i < NC; i += 64 * NT) {
+    double2 d = cz(0.0, 0.0);
+    if (i < N) d = crecip(chi0[q + (size_t)Nq * (shell + i + (size_t)Nc * w)]);
+    sh.dinv[i] = d;
+  }
+  if (tid == 0) sh.zero_pivot = 0;
+  __syncthreads();
+
+  // ---- M = A^T into the accumulators: M[i][c] = Gamma[c, i, w] + delta_ic / chi0[i]
+  // (identity on the padding).  Row i = 16 wave + 4g + j, column c = 16t + n.
+  // Rows and columns past N - 1 are clamped for the load and zeroed after it.
+  AccPlane<float, NT> Vr, Vi;
+  {
+    constexpr int LG = 2;
+    const char *Gb = reinterpret_cast<const char *>(G);
+    const int rb = 16 * wave + 4 * g;
+    unsigned rofs[4];
+    float dgx[4], dgy[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int i = rb + j;
+      rofs[j] = 16u * (unsigned)N * (unsigned)min(i, N - 1);
+      const double2 d = i < N ? sh.dinv[i] : cz(1.0, 0.0);
+      const bool here = n == 4 * g + j;
+      dgx[j] = here ? (float)d.x : 0.0f;
+      dgy[j] = here ? (float)d.y : 0.0f;
+    }
+#pragma unroll
+    for (int t0 = 0; t0 < NT; t0 += LG) {
+      int nl = n;
+      __asm__ volatile("" : "+v"(nl));
+      double2 v[LG][4];
+#pragma unroll
+      for (int tt = 0; tt < LG; ++tt) {
+        if (t0 + tt < NT) {
+          const unsigned cofs = 16u * (unsigned)min(16 * (t0 + tt) + nl, N - 1);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) v[tt][j] = *reinterpret_cast<const double2 *>(Gb + (cofs + rofs[j]));
+        }
+      }
+#pragma unroll
+      for (int tt = 0; tt < LG; ++tt) {
+        const int t = t0 + tt;
+        if (t < NT) {
+          const bool cok = 16 * t + n < N;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const bool ok = cok && rb + j < N;
+            float x = ok ? (float)v[tt][j].x : 0.0f, y = ok ? (float)v[tt][j].y : 0.0f;
+            if (t == wave) {
+              x += dgx[j];
+              y += dgy[j];
+            }
+            Vr.set(t, j, x);
+            Vi.set(t, j, y);
+          }
+          __asm__ volatile("" : "+v"(Vr.regs(t)), "+v"(Vi.regs(t)));
+        }
+      }
+      __asm__ volatile("" ::: "memory");
+    }
+  }
+
+  int zero_seen = 0;
+  int Lg = g, Ln = n, Ll = lane;
+
+  // this wave's rows -> sh.rows (the operand of the next step's phase 1)
+  auto publish_rows = [&]() {
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sh.rows[4 * Lg + j][16 * t + Ln] = make_float2(Vr.get(t, j), Vi.get(t, j));
+  };
+
+  // Invert D = this wave's tile kd (nr matrix rows; the rest is identity) by
+  // Gauss-Jordan steps with column pivoting inside D, implicit pivots; then
+  // scatter to sh.P in natural order: inv(D)[pc[r]][rowof[c]] = Dout[r][c].
+  auto invert_diag = [&](int kd, int nr) {
+    const v4 tr = Vr.select(kd), ti = Vi.select(kd);
+    float dr[4], di[4];
+    int pc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      dr[j] = tr[j];
+      di[j] = ti[j];
+      pc[j] = 4 * g + j;
+    }
+    int rw = n;             // the row that pivots on column n
+    bool pivd = n >= nr;    // column n pivoted (the identity columns never are candidates)
+#pragma unroll
+    for (int st = 0; st < 16; ++st) {
+      if (st < nr) {
+        const int gs = st >> 2, js = st & 3;
+        // argmax over D's unpivoted columns of row st (lane group gs, element
+        // js) as one 32-bit key: the float bits of |re| + |im| with the low
+        // byte 255 - column (ties go to the smaller column); pivoted columns
+        // have key 0.  A NaN has the largest key and propagates.
+        const float mag = fabsf(dr[js]) + fabsf(di[js]);
+        unsigned key = (__float_as_uint(mag) & 0xffffff00u) | (unsigned)(255 - n);
+        key = pivd ? 0u : key;
+        key = row_max16_u32(key);
+        const int p = 255 - (int)(__builtin_amdgcn_readlane((int)key, 16 * gs) & 0xff);
+        const float px = rdlane(dr[js], 16 * gs + p), py = rdlane(di[js], 16 * gs + p);
+        // the pivot row at this lane's column, the pivot column at this lane's rows
+        const float rx = bperm(16 * gs + Ln, dr[js]), ry = bperm(16 * gs + Ln, di[js]);
+        float cx[4], cy[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          cx[j] = bperm(16 * Lg + p, dr[j]);
+          cy[j] = bperm(16 * Lg + p, di[j]);
+        }
+        zero_seen |= (px == 0.0f && py == 0.0f);
+        // 1 / piv = conj(piv) / |piv|^2, reciprocal refined by Newton steps
+        const float dd = px * px + py * py;
+        float rd = fast_rcp(dd);
+        rd = fmaf(rd, fmaf(-dd, rd, 1.0f), rd);
+        rd = fmaf(rd, fmaf(-dd, rd, 1.0f), rd);
+        const float ivx = px * rd, ivy = -py * rd;
+        const float rsx = rx * ivx - ry * ivy, rsy = rx * ivy + ry * ivx;
+        const bool atp = n == p;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          float nx = fmaf(-cx[j], rsx, fmaf(cy[j], rsy, dr[j]));
+          float ny = fmaf(-cx[j], rsy, fmaf(-cy[j], rsx, di[j]));
+          const float qx = cx[j] * ivx - cy[j] * ivy, qy = cx[j] * ivy + cy[j] * ivx;
+          nx = atp ? qx : nx;
+          ny = atp ? qy : ny;
+          if (j == js) {
+            const bool own = g == gs;
+            nx = own ? (atp ? ivx : -rsx) : nx;
+            ny = own ? (atp ? ivy : -rsy) : ny;
+            pc[j] = own ? p : pc[j];
+          }
+          dr[j] = nx;
+          di[j] = ny;
+        }
+        pivd = pivd || atp;
+        rw = atp ? st : rw;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) sh.P[pc[j]][rw] = make_float2(dr[j], di[j]);
+  };
+
+  const int nr_of_last = N - 16 * (NB - 1);
+  if (wave == 0) {
+    __builtin_amdgcn_s_setprio(3);
+    publish_rows();
+    invert_diag(0, NB == 1 ? nr_of_last : 16);
+    __builtin_amdgcn_s_setprio(0);
+  }
+
+  for (int k = 0; k < NB; ++k) {
+    __asm__ volatile("" : "+v"(Lg), "+v"(Ln), "+v"(Ll));
+    __syncthreads();  // P and the owner's rows of block k published
+    // ---- phase 1: tile `wave` of W
+    {
+      v4 wr = (v4)0.0f, wi = (v4)0.0f;
+      if (wave == k) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float2 pv = sh.P[4 * Lg + j][Ln];
+          wr[j] = pv.x - ((4 * g + j) == n ? 1.0f : 0.0f);
+          wi[j] = pv.y;
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const float2 a = sh.P[Ln][4 * c + Lg];
+          const float2 b = sh.rows[4 * c + Lg][16 * wave + Ln];
+          wr = MT::mma(-a.x, b.x, wr);
+          wr = MT::mma(a.y, b.y, wr);
+          wi = MT::mma(-a.x, b.y, wi);
+          wi = MT::mma(-a.y, b.x, wi);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sh.Wb[Lg][wave][16 * j + Ln] = bf_pack_b(wr[j], wi[j]);
+    }
+    __syncthreads();  // W published
+    // ---- phase 2
+    const bool next_owner = wave == k + 1 && k + 1 < NB;
+    if (next_owner) __builtin_amdgcn_s_setprio(3);
+    if (wave == k) {
+      // the owner's rows become W + E_K
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const u32x4 u = sh.Wb[Lg][t][16 * j + Ln];
+          float x = unsplit_lo(u);
+          if (t == k && 4 * g + j == n) x += 1.0f;
+          Vr.set(t, j, x);
+          Vi.set(t, j, unsplit_hi(u));
+        }
+      }
+    } else {
+      // A operand: this wave's tile k (before the update), transposed
+      {
+        const v4 ar = Vr.select(k), ai = Vi.select(k);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sh.T[wave][4 * Lg + j][Ln] = make_float2(ar[j], ai[j]);
+      }
+      wave_sync_lds();
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const float2 a = sh.T[wave][Ln][4 * c + Lg];
+        const BfA o = bf_prep_a(a.x, a.y);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          v4 cr = Vr.tile(t), ci = Vi.tile(t);
+          bf_cmma(o, sh.Wb[c][t][Ll], cr, ci);
+          Vr.set_tile(t, cr);
+          Vi.set_tile(t, ci);
+        }
+      }
+    }
+    if (next_owner) {
+      publish_rows();
+      invert_diag(k + 1, k + 1 == NB - 1 ? nr_of_last : 16);
+      __builtin_amdgcn_s_setprio(0);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (zero_seen && lane == 0) sh.zero_pivot = 1;
+
+  // ---- y = X 1 (row sums) ------------------------------------------------------
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    float sx = 0.0f, sy = 0.0f;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      sx += Vr.get(t, j);
+      sy += Vi.get(t, j);
+    }
+    sx = row_sum16(sx);
+    sy = row_sum16(sy);
+    if (n == 0) sh.y[16 * wave + 4 * g + j] = cz((double)sx, (double)sy);
+  }
+  __syncthreads();
+
+  // ---- FP64 iterative refinement with the FP32 inverse ----------------------------
+  int it = 0;
+  double last_ratio = 0.0;
+  bool converged_stop = false;
+  for (; it < max_ir; ++it) {
+    // r_i = 1 - sum_j Gamma[j, i] y_j - y_i / chi0_i; wave w: rows 4k + g for k = w, w + NT, ...
+    for (int ib = 4 * wave; ib < N; ib += 4 * NT) {
+      const int i = ib + g;
+      double2 acc = cz(0.0, 0.0);
+      if (i < N) {
+        const double2 *col = G + (size_t)N * i;
+        double2 gv[NT];
+        if (N > 16 * (NT - 1)) {
+#pragma unroll
+          for (int k = 0; k < NT - 1; ++k) gv[k] = col[n + 16 * k];
+          gv[NT - 1] = (n + 16 * (NT - 1) < N) ? col[n + 16 * (NT - 1)] : cz(0.0, 0.0);
+        } else {
+#pragma unroll
+          for (int k = 0; k < NT; ++k) gv[k] = (n + 16 * k < N) ? col[n + 16 * k] : cz(0.0, 0.0);
+        }
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+          const double2 yk = sh.y[n + 16 * k];
+          acc.x = fma(gv[k].x, yk.x, fma(-gv[k].y, yk.y, acc.x));
+          acc.y = fma(gv[k].x, yk.y, fma(gv[k].y, yk.x, acc.y));
+        }
+      }
+      acc.x = row_sum16(acc.x);
+      acc.y = row_sum16(acc.y);
+      if (n == 0 && i < N) {
+        const double2 yi = sh.y[i];
+        const double2 d = cmul(yi, sh.dinv[i]);
+        sh.res[i] = make_float2((float)(1.0 - acc.x - d.x), (float)(-acc.y - d.y));
+      }
+    }
+    if (tid < NC - N) sh.res[N + tid] = make_float2(0.0f, 0.0f);
+    __syncthreads();
+    // dy = X r (FP32 products)
+    float dmax = 0.0f, ymax = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      float sx = 0.0f, sy = 0.0f;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const float2 rv = sh.res[16 * t + n];
+        sx = fmaf(Vr.get(t, j), rv.x, fmaf(-Vi.get(t, j), rv.y, sx));
+        sy = fmaf(Vr.get(t, j), rv.y, fmaf(Vi.get(t, j), rv.x, sy));
+      }
+      sx = row_sum16(sx);
+      sy = row_sum16(sy);
+      const int row = 16 * wave + 4 * g + j;
+      if (n == 0 && row < N) {
+        const double2 yo = sh.y[row];
+        const double2 yn = cz(yo.x + (double)sx, yo.y + (double)sy);
+        sh.y[row] = yn;
+        dmax = fmaxf(dmax, fabsf(sx) + fabsf(sy));
+        ymax = fmaxf(ymax, (float)(fabs(yn.x) + fabs(yn.y)));
+      }
+    }
+    const unsigned dk = wave_max_u32(__float_as_uint(dmax)), yk = wave_max_u32(__float_as_uint(ymax));
+    if (lane == 63) {
+      sh.redf[it & 1][0][wave] = __uint_as_float(dk);
+      sh.redf[it & 1][1][wave] = __uint_as_float(yk);
+    }
+    __syncthreads();
+    float dm = 0.0f, ym = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+      dm = fmaxf(dm, sh.redf[it & 1][0][k]);
+      ym = fmaxf(ym, sh.redf[it & 1][1][k]);
+    }
+    const double ratio = (double)dm / (double)ym;
+    const bool converged = !(ratio > 0x1p-50) || (it > 0 && ratio * ratio <= 0x1p-48 * last_ratio);
+    const bool stagnant = it > 0 && !(ratio < 0.5 * last_ratio);
+    const bool stop = converged || stagnant;
+    last_ratio = ratio;
+    if (stop) {
+      converged_stop = converged;
+      ++it;
+      break;
+    }
+  }
+  if (tid == 0) {
+    atomicMax(&counters[1], (unsigned long long)it);
+    atomicAdd(&counters[5], (unsigned long long)it);
+    if ((!converged_stop && !(last_ratio <= 0x1p-30)) || sh.zero_pivot) {
+      const unsigned long long kk = atomicAdd(&counters[2], 1ull);
+      if (redo) redo[kk] = lb;
+    }
+    unsigned long long bits;
+    const double lr = last_ratio == last_ratio ? last_ratio : 1.0;
+    memcpy(&bits, &lr, 8);
+    atomicMax(&counters[3], bits);
+  }
+
+  // ---- chi and gamma -------------------------------------------------------------
+  double part = 0.0;
+  for (int k = tid; k < N; k += 64 * NT) part += sh.y[k].x;
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+  if (lane == 0) sh.red[wave] = part;
+  __syncthreads();
+  double tot = 0.0;
+#pragma unroll
+  for (int k = 0; k < NT; ++k) tot += sh.red[k];
+  const double chi = tot / (beta * beta);
+  const double fac = 1.0 + s * U * chi;
+  for (int k = tid; k < N; k += 64 * NT) {
+    const double2 c0 = chi0[q + (size_t)Nq * (k + (size_t)Nc * w)];  // no shell offset (BSETools.jl:236)
+    gamma_out[q + (size_t)Nq * (k + (size_t)N * w)] = cdiv(sh.y[k], cscale(c0, fac));
+  }
+  if (tid == 0) {
+    chi_out[q + (size_t)Nq * w] = chi;
+    if (sh.zero_pivot && !redo) atomicAdd(&counters[0], 1ull);
+  }
+}
+
+template <int NT>
+__global__ __launch_bounds__(64 * NT, (bgj_min_waves<NT>())) void bse_bgj_kernel(
+    int N, int Nq, int Nc, int shell, const double2 *__restrict__ Gam, const double2 *__restrict__ chi0, double s,
+    double U, double beta, int max_ir, double *__restrict__ chi_out, double2 *__restrict__ gamma_out,
+    unsigned long long *__restrict__ counters, int *__restrict__ redo) {
+  __shared__ BgjShared<NT> sh;
+  bgj_solve<NT>(sh, xcd_logical_block(blockIdx.x, gridDim.x), N, Nq, Nc, shell, Gam, chi0, s, U, beta, max_ir,
+                chi_out, gamma_out, counters, redo);
+}
+
+template <int NT>
+int run_bgj(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam, const double2 *chi0,
+            double s, double U, double beta, int max_ir, double *chi, double2 *gamma, unsigned long long *cnt,
+            int *redo, hipStream_t st) {
+  if (N > 16 * NT) return ldga_fail(ctx, LDGA_ERR_SHAPE, "BSE tile too small for N=%d", N);
+  const unsigned nblocks = (unsigned)Nq * Nw;
+  if (nblocks == 0) return LDGA_OK;
+  bse_bgj_kernel<NT><<<dim3(nblocks), dim3(64 * NT), 0, st>>>(N, Nq, Nc, shell, Gam, chi0, s, U, beta, max_ir, chi,
+                                                             gamma, cnt, redo);
+  LDGA_HIP_TRY(ctx, hipGetLastError());
+  return LDGA_OK;
+}
+
+}  // namespace
+
+// Mixed, block-pivoted: FP32 16-row blocked Gauss-Jordan on MFMA + FP64
+// refinement, Nnu <= 160.  With redo != NULL (room for Nq Nw ints) the matrices
+// whose refinement failed (or whose diagonal block met a zero pivot) are listed
+// there; counters[2] holds their number.
+int launch_bse_mixed_bgj(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam,
+                         const double2 *chi0, double s, double U, double beta, double *chi, double2 *gamma,
+                         unsigned long long *cnt, int *redo, hipStream_t st) {
+  const int nt = (N + 15) / 16, ir = 6;
+#define LDGA_BGJ(NT_) return run_bgj<NT_>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo, st)
+  switch (nt) {
+    case 1:
+    case 2:
+    case 3: LDGA_BGJ(3);
+    case 4: LDGA_BGJ(4);
+    case 5: LDGA_BGJ(5);
+    case 6: LDGA_BGJ(6);
+    case 7: LDGA_BGJ(7);
+    case 8: LDGA_BGJ(8);
+    case 9: LDGA_BGJ(9);
+    case 10: LDGA_BGJ(10);
+    default: return ldga_fail(ctx, LDGA_ERR_SHAPE, "mixed BSE supports Nnu <= 160 (got %d)", N);
+  }
+#undef LDGA_BGJ
+}

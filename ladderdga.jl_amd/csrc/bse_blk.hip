@@ -772,9 +772,16 @@ int launch_bse_blk_f64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, 
 // Mixed: FP32 inverse on MFMA + FP64 refinement, Nnu <= 160.  With redo !=
 // NULL (room for Nq Nw ints) the matrices whose refinement failed are listed
 // there; counters[2] holds their number.
+int launch_bse_mixed_bgj(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam,
+                         const double2 *chi0, double s, double U, double beta, double *chi, double2 *gamma,
+                         unsigned long long *cnt, int *redo, hipStream_t st);
+
 int launch_bse_mixed(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam, const double2 *chi0,
                      double s, double U, double beta, double *chi, double2 *gamma, unsigned long long *cnt,
                      int *redo, hipStream_t st) {
+  const char *e = getenv("LDGA_BSE_MIXED");
+  if (!(e && strcmp(e, "panel") == 0))
+    return launch_bse_mixed_bgj(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, chi, gamma, cnt, redo, st);
   const int nt = (N + 15) / 16, ir = 6;
 #define LDGA_BLK_MIXED(NT_)                                                                                   \
   return run_blk<float, NT_, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo, \
