@@ -47,6 +47,7 @@
 // or the correction stops halving (the FP64 residual's rounding level).
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "ldga_internal.h"
 
@@ -83,6 +84,30 @@ __device__ __forceinline__ unsigned row_max16_u32(unsigned k) {
 // the value of v in lane src (any lane of the wave)
 __device__ __forceinline__ float bperm(int src, float v) {
   return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
+}
+
+// f(std::integral_constant<int, I>) for I = B .. E-1 (compile-time indices)
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// the value of v in lane L of each 16-lane row, in every lane of the row (DPP row_newbcast)
+template <int L>
+__device__ __forceinline__ float row_bcast(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x150 + L, 0xf, 0xf, false));
+}
+
+// per lane: b where bit lane of the (wave-uniform) mask is set, else a.  A
+// select with a constant lane mask, kept a select (the compiler otherwise
+// turns a lane-dependent select feeding arithmetic into a divergent branch).
+__device__ __forceinline__ float vsel(float a, float b, unsigned long long mask) {
+  float r;
+  __asm__("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(mask));
+  return r;
 }
 
 // an FP32 value from its packed split (h + m + l, exact)
@@ -186,152 +211,132 @@ __device__ __forceinline__ void bgj_solve(BgjShared<NT> &sh, const int lb, const
       for (int j = 0; j < 4; ++j) sh.rows[4 * Lg + j][16 * t + Ln] = make_float2(Vr.get(t, j), Vi.get(t, j));
   };
 
-  // Invert D = this wave's tile kd (nr matrix rows; the rest is identity) by
-  // Gauss-Jordan steps with column pivoting inside D, implicit pivots; then
-  // scatter to sh.P in natural order: inv(D)[pc[r]][rowof[c]] = Dout[r][c].
-  auto invert_diag = [&](int kd, int nr) {
+  // Invert D = this wave's tile kd in place by Gauss-Jordan steps on its
+  // diagonal (step st pivots on D[st][st], iv = 1/D[st][st]):
+  //   row st   <- (D[st][:] with D[st][st] := 1) * iv
+  //   row i    <- (D[i][:] with D[i][st] := 0) - D[i][st] * (new row st)    (i != st)
+  // and write inv(D) to sh.P.  Every lane exchange has a fixed source: the
+  // pivot by readlane, the pivot row from lane group st/4 (a lane permute
+  // issued at the start of the step), the pivot column from lane st of each
+  // row (DPP).  In the last block the rows
+  // past N - 1 are identity rows with zeros in the matrix columns: their
+  // steps have piv = 1 and change nothing (exact products by 0 and 1).
+  auto invert_diag = [&](int kd) {
     const v4 tr = Vr.select(kd), ti = Vi.select(kd);
     float dr[4], di[4];
-    int pc[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       dr[j] = tr[j];
       di[j] = ti[j];
-      pc[j] = 4 * g + j;
     }
-    int rw = n;             // the row that pivots on column n
-    bool pivd = n >= nr;    // column n pivoted (the identity columns never are candidates)
+    static_for<0, 16>([&](auto ST) {
+      constexpr int st = decltype(ST)::value, gs = st >> 2, js = st & 3;
+      // lanes holding column st (one per row) and lane group gs (row st)
+      constexpr unsigned long long atp = 0x0001000100010001ull << st, own = 0xffffull << (16 * gs);
+      const float rx = bperm(16 * gs + Ln, dr[js]), ry = bperm(16 * gs + Ln, di[js]);  // pivot row, column n
+      const float px = rdlane(dr[js], 16 * gs + st), py = rdlane(di[js], 16 * gs + st);
+      float cx[4], cy[4];  // pivot column, rows 4g + j
 #pragma unroll
-    for (int st = 0; st < 16; ++st) {
-      if (st < nr) {
-        const int gs = st >> 2, js = st & 3;
-        // argmax over D's unpivoted columns of row st (lane group gs, element
-        // js) as one 32-bit key: the float bits of |re| + |im| with the low
-        // byte 255 - column (ties go to the smaller column); pivoted columns
-        // have key 0.  A NaN has the largest key and propagates.
-        const float mag = fabsf(dr[js]) + fabsf(di[js]);
-        unsigned key = (__float_as_uint(mag) & 0xffffff00u) | (unsigned)(255 - n);
-        key = pivd ? 0u : key;
-        key = row_max16_u32(key);
-        const int p = 255 - (int)(__builtin_amdgcn_readlane((int)key, 16 * gs) & 0xff);
-        const float px = rdlane(dr[js], 16 * gs + p), py = rdlane(di[js], 16 * gs + p);
-        // the pivot row at this lane's column, the pivot column at this lane's rows
-        const float rx = bperm(16 * gs + Ln, dr[js]), ry = bperm(16 * gs + Ln, di[js]);
-        float cx[4], cy[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          cx[j] = bperm(16 * Lg + p, dr[j]);
-          cy[j] = bperm(16 * Lg + p, di[j]);
-        }
-        zero_seen |= (px == 0.0f && py == 0.0f);
-        // 1 / piv = conj(piv) / |piv|^2, reciprocal refined by Newton steps
-        const float dd = px * px + py * py;
-        float rd = fast_rcp(dd);
-        rd = fmaf(rd, fmaf(-dd, rd, 1.0f), rd);
-        rd = fmaf(rd, fmaf(-dd, rd, 1.0f), rd);
-        const float ivx = px * rd, ivy = -py * rd;
-        const float rsx = rx * ivx - ry * ivy, rsy = rx * ivy + ry * ivx;
-        const bool atp = n == p;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          float nx = fmaf(-cx[j], rsx, fmaf(cy[j], rsy, dr[j]));
-          float ny = fmaf(-cx[j], rsy, fmaf(-cy[j], rsx, di[j]));
-          const float qx = cx[j] * ivx - cy[j] * ivy, qy = cx[j] * ivy + cy[j] * ivx;
-          nx = atp ? qx : nx;
-          ny = atp ? qy : ny;
-          if (j == js) {
-            const bool own = g == gs;
-            nx = own ? (atp ? ivx : -rsx) : nx;
-            ny = own ? (atp ? ivy : -rsy) : ny;
-            pc[j] = own ? p : pc[j];
-          }
-          dr[j] = nx;
-          di[j] = ny;
-        }
-        pivd = pivd || atp;
-        rw = atp ? st : rw;
+      for (int j = 0; j < 4; ++j) {
+        cx[j] = row_bcast<st>(dr[j]);
+        cy[j] = row_bcast<st>(di[j]);
       }
-    }
+      // 1 / piv = conj(piv) / |piv|^2, reciprocal refined by a Newton step
+      const float dd = px * px + py * py;
+      zero_seen |= dd == 0.0f;
+      float rd = fast_rcp(dd);
+      rd = fmaf(rd, fmaf(-dd, rd, 1.0f), rd);
+      const float ivx = px * rd, ivy = -py * rd;
+      const float ax = vsel(rx, 1.0f, atp), ay = vsel(ry, 0.0f, atp);
+      const float nrx = ax * ivx - ay * ivy, nry = ax * ivy + ay * ivx;  // new row st at column n
 #pragma unroll
-    for (int j = 0; j < 4; ++j) sh.P[pc[j]][rw] = make_float2(dr[j], di[j]);
+      for (int j = 0; j < 4; ++j) {
+        float x = vsel(dr[j], 0.0f, atp), y = vsel(di[j], 0.0f, atp);
+        x = fmaf(-cx[j], nrx, fmaf(cy[j], nry, x));
+        y = fmaf(-cx[j], nry, fmaf(-cy[j], nrx, y));
+        if (j == js) {
+          x = vsel(x, nrx, own);
+          y = vsel(y, nry, own);
+        }
+        dr[j] = x;
+        di[j] = y;
+      }
+    });
+#pragma unroll
+    for (int j = 0; j < 4; ++j) sh.P[4 * Lg + j][Ln] = make_float2(dr[j], di[j]);
   };
-
-  const int nr_of_last = N - 16 * (NB - 1);
-  if (wave == 0) {
-    __builtin_amdgcn_s_setprio(3);
-    publish_rows();
-    invert_diag(0, NB == 1 ? nr_of_last : 16);
-    __builtin_amdgcn_s_setprio(0);
-  }
-
-  for (int k = 0; k < NB; ++k) {
+  // k = -1: only the owner of block 0 publishes its rows and inverts its block
+  for (int k = -1; k < NB; ++k) {
     __asm__ volatile("" : "+v"(Lg), "+v"(Ln), "+v"(Ll));
-    __syncthreads();  // P and the owner's rows of block k published
-    // ---- phase 1: tile `wave` of W
-    {
-      v4 wr = (v4)0.0f, wi = (v4)0.0f;
-      if (wave == k) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const float2 pv = sh.P[4 * Lg + j][Ln];
-          wr[j] = pv.x - ((4 * g + j) == n ? 1.0f : 0.0f);
-          wi[j] = pv.y;
-        }
-      } else {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const float2 a = sh.P[Ln][4 * c + Lg];
-          const float2 b = sh.rows[4 * c + Lg][16 * wave + Ln];
-          wr = MT::mma(-a.x, b.x, wr);
-          wr = MT::mma(a.y, b.y, wr);
-          wi = MT::mma(-a.x, b.y, wi);
-          wi = MT::mma(-a.y, b.x, wi);
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < 4; ++j) sh.Wb[Lg][wave][16 * j + Ln] = bf_pack_b(wr[j], wi[j]);
-    }
-    __syncthreads();  // W published
-    // ---- phase 2
     const bool next_owner = wave == k + 1 && k + 1 < NB;
-    if (next_owner) __builtin_amdgcn_s_setprio(3);
-    if (wave == k) {
-      // the owner's rows become W + E_K
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const u32x4 u = sh.Wb[Lg][t][16 * j + Ln];
-          float x = unsplit_lo(u);
-          if (t == k && 4 * g + j == n) x += 1.0f;
-          Vr.set(t, j, x);
-          Vi.set(t, j, unsplit_hi(u));
-        }
-      }
-    } else {
-      // A operand: this wave's tile k (before the update), transposed
+    if (k >= 0) {
+      __syncthreads();  // P and the owner's rows of block k published
+      // ---- phase 1: tile `wave` of W
       {
-        const v4 ar = Vr.select(k), ai = Vi.select(k);
+        v4 wr = (v4)0.0f, wi = (v4)0.0f;
+        if (wave == k) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) sh.T[wave][4 * Lg + j][Ln] = make_float2(ar[j], ai[j]);
+          for (int j = 0; j < 4; ++j) {
+            const float2 pv = sh.P[4 * Lg + j][Ln];
+            wr[j] = pv.x - ((4 * g + j) == n ? 1.0f : 0.0f);
+            wi[j] = pv.y;
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const float2 a = sh.P[Ln][4 * c + Lg];
+            const float2 b = sh.rows[4 * c + Lg][16 * wave + Ln];
+            wr = MT::mma(-a.x, b.x, wr);
+            wr = MT::mma(a.y, b.y, wr);
+            wi = MT::mma(-a.x, b.y, wi);
+            wi = MT::mma(-a.y, b.x, wi);
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sh.Wb[Lg][wave][16 * j + Ln] = bf_pack_b(wr[j], wi[j]);
       }
-      wave_sync_lds();
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const float2 a = sh.T[wave][Ln][4 * c + Lg];
-        const BfA o = bf_prep_a(a.x, a.y);
+      __syncthreads();  // W published
+      // ---- phase 2
+      if (next_owner) __builtin_amdgcn_s_setprio(3);
+      if (wave == k) {
+        // the owner's rows become W + E_K
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-          v4 cr = Vr.tile(t), ci = Vi.tile(t);
-          bf_cmma(o, sh.Wb[c][t][Ll], cr, ci);
-          Vr.set_tile(t, cr);
-          Vi.set_tile(t, ci);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const u32x4 u = sh.Wb[Lg][t][16 * j + Ln];
+            float x = unsplit_lo(u);
+            if (t == k && 4 * g + j == n) x += 1.0f;
+            Vr.set(t, j, x);
+            Vi.set(t, j, unsplit_hi(u));
+          }
+        }
+      } else {
+        // A operand: this wave's tile k (before the update), transposed
+        {
+          const v4 ar = Vr.select(k), ai = Vi.select(k);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) sh.T[wave][4 * Lg + j][Ln] = make_float2(ar[j], ai[j]);
+        }
+        wave_sync_lds();
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const float2 a = sh.T[wave][Ln][4 * c + Lg];
+          const BfA o = bf_prep_a(a.x, a.y);
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+            v4 cr = Vr.tile(t), ci = Vi.tile(t);
+            bf_cmma(o, sh.Wb[c][t][Ll], cr, ci);
+            Vr.set_tile(t, cr);
+            Vi.set_tile(t, ci);
+          }
         }
       }
     }
     if (next_owner) {
+      __builtin_amdgcn_s_setprio(3);
       publish_rows();
-      invert_diag(k + 1, k + 1 == NB - 1 ? nr_of_last : 16);
+      invert_diag(k + 1);
       __builtin_amdgcn_s_setprio(0);
     }
     __builtin_amdgcn_wave_barrier();
@@ -425,7 +430,7 @@ __device__ __forceinline__ void bgj_solve(BgjShared<NT> &sh, const int lb, const
       ym = fmaxf(ym, sh.redf[it & 1][1][k]);
     }
     const double ratio = (double)dm / (double)ym;
-    const bool converged = !(ratio > 0x1p-50) || (it > 0 && ratio * ratio <= 0x1p-48 * last_ratio);
+    const bool converged = ratio <= 0x1p-50 || (it > 0 && ratio * ratio <= 0x1p-48 * last_ratio);
     const bool stagnant = it > 0 && !(ratio < 0.5 * last_ratio);
     const bool stop = converged || stagnant;
     last_ratio = ratio;
