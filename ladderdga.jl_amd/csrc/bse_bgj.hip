@@ -60,16 +60,19 @@ using namespace ldga_bse;
 template <int NT>
 struct BgjShared {
   static constexpr int NC = 16 * NT;
-  float2 rows[16][NC + 1];  // the step owner's rows, as of the previous step
+  // the step owner's rows as of the previous step, tile column w written by
+  // wave w; in phase 2 wave w also uses its tile column as transposition scratch
+  float2 rows[16][NC + 1];
+  float2 R[16][NC + 1];     // the next owner's rows before the step's update
   u32x4 Wb[4][NT][64];      // W split for the bf16 B operand: [chunk][tile][lane (k row = lane / 16)]
   float2 P[16][17];         // inv(D) of the step's diagonal block, natural order
-  float2 T[NT][16][17];     // per wave: its tile K transposed into the A operand layout
   double2 y[NC];            // solution (FP64)
   double2 dinv[NC];         // 1/chi0 on the diagonal (FP64)
   float2 res[NC];           // refinement residual (rounded to FP32 for X r)
   double red[NT];
   float redf[2][2][NT];
   int zero_pivot;
+  int r_ready;  // the step whose R is published (set by its owner, read by the other waves)
 };
 
 // maximum over the 16 lanes of each DPP row, in every lane of the row
@@ -119,6 +122,21 @@ __device__ __forceinline__ float unsplit_hi(u32x4 u) { return (bf_hi(u[0]) + bf_
 template <int NT>
 constexpr int bgj_min_waves() { return NT <= 8 ? 4 : 1; }
 
+// tile tq (wave-uniform runtime index) of an accumulator plane, picked by
+// scalar branches (no runtime-indexed access to the register array)
+template <typename P>
+__device__ __forceinline__ mfma_traits<float>::v4 tile_of(const P &pl, int tq) {
+  tq = __builtin_amdgcn_readfirstlane(tq);
+  mfma_traits<float>::v4 r = pl.tile(0);
+#pragma unroll
+  for (int t = 1; t < P::ntiles; ++t)
+    if (tq == t) {
+      r = pl.tile(t);
+      __asm__ volatile("" : "+v"(r));
+    }
+  return r;
+}
+
 // One workgroup solves matrix lb = q + Nq w.
 template <int NT>
 __device__ __forceinline__ void bgj_solve(BgjShared<NT> &sh, const int lb, const int N, const int Nq, const int Nc,
@@ -142,7 +160,10 @@ __device__ __forceinline__ void bgj_solve(BgjShared<NT> &sh, const int lb, const
     if (i < N) d = crecip(chi0[q + (size_t)Nq * (shell + i + (size_t)Nc * w)]);
     sh.dinv[i] = d;
   }
-  if (tid == 0) sh.zero_pivot = 0;
+  if (tid == 0) {
+    sh.zero_pivot = 0;
+    sh.r_ready = -1;
+  }
   __syncthreads();
 
   // ---- M = A^T into the accumulators: M[i][c] = Gamma[c, i, w] + delta_ic / chi0[i]
@@ -204,6 +225,14 @@ __device__ __forceinline__ void bgj_solve(BgjShared<NT> &sh, const int lb, const
   int Lg = g, Ln = n, Ll = lane;
 
   // this wave's rows -> sh.rows (the operand of the next step's phase 1)
+  // this wave's tile kt, transposed into its own tile column of sh.rows
+  // (A operand: row n, column 4c + g read back as sh.rows[n][16 wave + 4c + g])
+  auto transpose_tile = [&](int kt) {
+    const v4 ar = Vr.select(kt), ai = Vi.select(kt);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) sh.rows[4 * Lg + j][16 * wave + Ln] = make_float2(ar[j], ai[j]);
+    wave_sync_lds();
+  };
   auto publish_rows = [&]() {
 #pragma unroll
     for (int t = 0; t < NT; ++t)
@@ -211,7 +240,7 @@ __device__ __forceinline__ void bgj_solve(BgjShared<NT> &sh, const int lb, const
       for (int j = 0; j < 4; ++j) sh.rows[4 * Lg + j][16 * t + Ln] = make_float2(Vr.get(t, j), Vi.get(t, j));
   };
 
-  // Invert D = this wave's tile kd in place by Gauss-Jordan steps on its
+  // Invert D = (tr, ti), a tile of this wave, in place by Gauss-Jordan steps on its
   // diagonal (step st pivots on D[st][st], iv = 1/D[st][st]):
   //   row st   <- (D[st][:] with D[st][st] := 1) * iv
   //   row i    <- (D[i][:] with D[i][st] := 0) - D[i][st] * (new row st)    (i != st)
@@ -221,8 +250,7 @@ __device__ __forceinline__ void bgj_solve(BgjShared<NT> &sh, const int lb, const
   // row (DPP).  In the last block the rows
   // past N - 1 are identity rows with zeros in the matrix columns: their
   // steps have piv = 1 and change nothing (exact products by 0 and 1).
-  auto invert_diag = [&](int kd) {
-    const v4 tr = Vr.select(kd), ti = Vi.select(kd);
+  auto invert_diag = [&](const v4 tr, const v4 ti) {
     float dr[4], di[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -297,46 +325,86 @@ __device__ __forceinline__ void bgj_solve(BgjShared<NT> &sh, const int lb, const
       }
       __syncthreads();  // W published
       // ---- phase 2
-      if (next_owner) __builtin_amdgcn_s_setprio(3);
-      if (wave == k) {
-        // the owner's rows become W + E_K
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const u32x4 u = sh.Wb[Lg][t][16 * j + Ln];
-            float x = unsplit_lo(u);
-            if (t == k && 4 * g + j == n) x += 1.0f;
-            Vr.set(t, j, x);
-            Vi.set(t, j, unsplit_hi(u));
-          }
-        }
-      } else {
-        // A operand: this wave's tile k (before the update), transposed
-        {
-          const v4 ar = Vr.select(k), ai = Vi.select(k);
-#pragma unroll
-          for (int j = 0; j < 4; ++j) sh.T[wave][4 * Lg + j][Ln] = make_float2(ar[j], ai[j]);
-        }
-        wave_sync_lds();
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const float2 a = sh.T[wave][Ln][4 * c + Lg];
-          const BfA o = bf_prep_a(a.x, a.y);
+      if (!next_owner) {
+        if (wave == k) {
+          // the owner's rows become W + E_K
 #pragma unroll
           for (int t = 0; t < NT; ++t) {
-            v4 cr = Vr.tile(t), ci = Vi.tile(t);
-            bf_cmma(o, sh.Wb[c][t][Ll], cr, ci);
-            Vr.set_tile(t, cr);
-            Vi.set_tile(t, ci);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const u32x4 u = sh.Wb[Lg][t][16 * j + Ln];
+              float x = unsplit_lo(u);
+              if (t == k && 4 * g + j == n) x += 1.0f;
+              Vr.set(t, j, x);
+              Vi.set(t, j, unsplit_hi(u));
+            }
           }
+        } else {
+          // M[I,:] += M[I,K] W; A operand: this wave's tile k (before the update), transposed
+          transpose_tile(k);
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const float2 a = sh.rows[Ln][16 * wave + 4 * c + Lg];
+            const BfA o = bf_prep_a(a.x, a.y);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+              v4 cr = Vr.tile(t), ci = Vi.tile(t);
+              bf_cmma(o, sh.Wb[c][t][Ll], cr, ci);
+              Vr.set_tile(t, cr);
+              Vi.set_tile(t, ci);
+            }
+          }
+        }
+        if (k + 1 < NB) {
+          // tile column `wave` of the next owner's rows, up to date:
+          //   rows[:, wave] = R[:, wave] + R[:, k] W[:, wave]
+          // R is published at the start of the next owner's phase 2; the wait
+          // is bounded, and a matrix whose wait ran out is re-solved in FP64
+          int spins = 0;
+          while (*(volatile int *)&sh.r_ready != k + 1 && ++spins < (1 << 22)) __builtin_amdgcn_s_sleep(1);
+          if (spins >= (1 << 22)) zero_seen = 1;
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+          v4 cr, ci;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const float2 v = sh.R[4 * Lg + j][16 * wave + Ln];
+            cr[j] = v.x;
+            ci[j] = v.y;
+          }
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const float2 a = sh.R[Ln][16 * k + 4 * c + Lg];
+            bf_cmma(bf_prep_a(a.x, a.y), sh.Wb[c][wave][Ll], cr, ci);
+          }
+#pragma unroll
+          for (int j = 0; j < 4; ++j) sh.rows[4 * Lg + j][16 * wave + Ln] = make_float2(cr[j], ci[j]);
         }
       }
     }
     if (next_owner) {
+      // the owner of block k+1 (k = -1: block 0, its rows as loaded): publish
+      // R (its rows before this step's update; the other waves bring them up
+      // to date, one tile column each), bring only its diagonal tile up to
+      // date, and invert it
       __builtin_amdgcn_s_setprio(3);
-      publish_rows();
-      invert_diag(k + 1);
+      v4 dtr = Vr.select(k + 1), dti = Vi.select(k + 1);
+      if (k >= 0) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) sh.R[4 * Lg + j][16 * t + Ln] = make_float2(Vr.get(t, j), Vi.get(t, j));
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (lane == 0) *(volatile int *)&sh.r_ready = k + 1;
+        transpose_tile(k);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const float2 a = sh.rows[Ln][16 * wave + 4 * c + Lg];
+          bf_cmma(bf_prep_a(a.x, a.y), sh.Wb[c][k + 1][Ll], dtr, dti);
+        }
+      } else {
+        publish_rows();
+      }
+      invert_diag(dtr, dti);
       __builtin_amdgcn_s_setprio(0);
     }
     __builtin_amdgcn_wave_barrier();

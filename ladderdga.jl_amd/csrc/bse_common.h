@@ -231,6 +231,7 @@ struct AccPlane;
 template <typename T, int NT>
 struct AccPlane<T, NT, true> {
   static constexpr bool bank = true;
+  static constexpr int ntiles = NT;
   typedef typename mfma_traits<T>::v4 v4;
   typedef T bank_t __attribute__((ext_vector_type(NT <= 4 ? 16 : 32)));
   bank_t b;
@@ -262,6 +263,7 @@ struct AccPlane<T, NT, true> {
 template <typename T, int NT>
 struct AccPlane<T, NT, false> {
   static constexpr bool bank = false;
+  static constexpr int ntiles = NT;
   typedef typename mfma_traits<T>::v4 v4;
   v4 v[NT];
   __device__ __forceinline__ void zero() {
