@@ -146,8 +146,7 @@ int ldga_chi_gamma_dev(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr,
  * No counterpart in the reference.  ldga_chi_gamma_dev without any device
  * allocation, for callers that cannot allocate: code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc.  Same kernels and
- * results as ldga_chi_gamma_dev (FP64: the blocked kernel, whatever
- * LDGA_BSE_F64 says); its scratch (report counters, the mixed re-solve list)
+ * results as ldga_chi_gamma_dev; its scratch (report counters, the mixed re-solve list)
  * is a static device array of 4 MiB + 256 bytes per GPU, held until the process
  * ends.  Limits: Nnu <= 128 and Nq (2 n_iw + 1) <= 1048576 matrices, else
  * LDGA_ERR_SHAPE.  All calls on one GPU share the array: calls that overlap on

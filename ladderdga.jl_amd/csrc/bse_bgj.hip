@@ -571,9 +571,9 @@ int run_bgj(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const doubl
 // refinement, Nnu <= 160.  With redo != NULL (room for Nq Nw ints) the matrices
 // whose refinement failed (or whose diagonal block met a zero pivot) are listed
 // there; counters[2] holds their number.
-int launch_bse_mixed_bgj(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam,
-                         const double2 *chi0, double s, double U, double beta, double *chi, double2 *gamma,
-                         unsigned long long *cnt, int *redo, hipStream_t st) {
+int launch_bse_mixed(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam,
+                     const double2 *chi0, double s, double U, double beta, double *chi, double2 *gamma,
+                     unsigned long long *cnt, int *redo, hipStream_t st) {
   const int nt = (N + 15) / 16, ir = 6;
 #define LDGA_BGJ(NT_) return run_bgj<NT_>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo, st)
   switch (nt) {

@@ -29,13 +29,9 @@
 // the complex product split into four real K=4 products (4M scheme).  The
 // panel ownership rotates over the waves.
 //
-// T = double: the FP64 path (f64 16x16x4 MFMA).  T = float with iterative
-// refinement: the mixed path -- the FP32 inverse X is kept in the
-// accumulators and y is refined in FP64:  r = 1 - A^T y  (Gamma, chi0 in FP64),
-// y += X r, until |X r|_inf <= 2^-50 |y|_inf (or the next correction,
-// extrapolated from the contraction rate, is <= 2^-48 |y|_inf, a few units in
-// the last place) or the correction stops halving (the FP64 residual's
-// rounding level, kappa(A) 2^-53, is reached).
+// The FP64 path (f64 16x16x4 MFMA): its pivot sequence is LAPACK's (izamax
+// over the whole row, up to ties), so its rounding tracks zgetrf+zgetri.  The
+// mixed path is bse_bgj.hip.
 #include <algorithm>
 #include <climits>
 #include <cstring>
@@ -49,11 +45,6 @@ namespace {
 
 using namespace ldga_bse;
 
-// smallest FP32 tile count that runs the rank-4 updates on bf16 MFMA: below it
-// the workgroups are few per CU and the factoring chain, which the bf16 form
-// lengthens by packing the B operand, is the critical path
-constexpr int kBfMinNT = 6;
-
 template <typename T, int NT>
 struct BlkShared {
   typedef typename mfma_traits<T>::c2 c2;
@@ -61,17 +52,8 @@ struct BlkShared {
   // rows padded so that rows 1024 bytes apart (NC = 128) do not fall on the
   // same LDS banks (the column reads of S and the per-group reads of W)
   static constexpr int SPAD = sizeof(T) == 8 ? 4 : 2, WPAD = sizeof(T) == 8 ? 32 : 16;
-  // FP32 from NT = kBfMinNT: the rank-4 updates run on bf16 MFMA with the split
-  // B operand published in Wb (BF), and to make room for it only the panel's
-  // four rows are staged (S4); otherwise the owner stages all sixteen of its rows
-  static constexpr bool BF = std::is_same<T, float>::value && NT >= kBfMinNT;
-  static constexpr bool S4 = BF;
   c2 W[3][4][NC + WPAD];   // published panel update (triple-buffered)
-  // FP32: W as the bf16 MFMA B operand (bf_pack_b), published with it
-  u32x4 Wb[BF ? 3 : 1][4][BF ? NC + 1 : 1];
-  // staged by a panel's owner (triple-buffered): FP32 the panel's four rows
-  // (row r: lane group r), FP64 all sixteen rows of the owner
-  c2 S[3][S4 ? 4 : 16][NC + SPAD];
+  c2 S[3][16][NC + SPAD];  // all sixteen rows of a panel's owner, staged (triple-buffered)
   T Cr[NT][16][4];     // per wave: own rows at the four pivot columns (real)
   T Ci[NT][16][4];     // (imaginary)
   int P[3][4];         // pivot columns of the panel
@@ -79,25 +61,16 @@ struct BlkShared {
   int pcol_of_row[NC]; // pivot column of the row at each accumulator position
   double2 y[NC];       // solution (FP64)
   double2 dinv[NC];    // 1/chi0 on the diagonal (FP64)
-  c2 res[NC];          // refinement residual (rounded to T for X r), at the pivot column of its row
-  double red[2][NT];
-  float redf[2][2][NT];
+  double red[NT];
   int zero_pivot;
 };
 
-// ---------------------------------------------------------------------------
-// FP32 instances up to NT = 8 ask for four waves per SIMD (<= 128 VGPRs) so
-// that two workgroups share a CU and overlap their serial pivot chains.
-template <typename T, int NT>
-constexpr int blk_min_waves() { return (sizeof(T) == 4 && NT <= 8) ? 4 : 1; }
-
 // One workgroup solves matrix lb = q + Nq w (the body of both kernels below).
-template <typename T, int NT, bool REFINE>
+template <typename T, int NT>
 __device__ __forceinline__ void blk_solve(BlkShared<T, NT> &sh, const int lb, int N, int Nq, int Nc, int shell,
                                           const double2 *__restrict__ Gam, const double2 *__restrict__ chi0, double s,
-                                          double U, double beta, int max_ir, double *__restrict__ chi_out,
-                                          double2 *__restrict__ gamma_out, unsigned long long *__restrict__ counters,
-                                          int *__restrict__ redo) {
+                                          double U, double beta, double *__restrict__ chi_out,
+                                          double2 *__restrict__ gamma_out, unsigned long long *__restrict__ counters) {
   typedef mfma_traits<T> MT;
   typedef typename MT::v4 v4;
   typedef typename MT::c2 c2;
@@ -139,7 +112,7 @@ __device__ __forceinline__ void blk_solve(BlkShared<T, NT> &sh, const int lb, in
   // Row i's diagonal lies in tile (j NT + wave) / 4, the same for the whole
   // wave, at lane i % 16: it is added there only, under a scalar branch.
   AccPlane<T, NT> Vr, Vi;  // every element is set below
-  constexpr int LG = sizeof(T) == 4 ? 2 : 4;
+  constexpr int LG = 4;
   const char *Gb = reinterpret_cast<const char *>(G);
   int tdiag[4];
   unsigned rofs[4];  // byte offset of row i (clamped) in Gamma(w)
@@ -214,25 +187,13 @@ __device__ __forceinline__ void blk_solve(BlkShared<T, NT> &sh, const int lb, in
   // keeping (and spilling) one hoisted register per array element.
   int Lg = g, Ln = n, Ll = lane;
 
-  // the rows of this wave (the owner of panel m) -> staging buffer m % 3:
-  // FP32 the four panel rows (element m / NT of each lane group; row r = lane
-  // group r), FP64 all sixteen (tile row order)
+  // all sixteen rows of this wave (the owner of panel m) -> staging buffer m % 3 (tile row order)
   auto stage_rows = [&](int m) {
     const int sb = m % 3;
-    if constexpr (!BlkShared<T, NT>::S4) {
 #pragma unroll
-      for (int t = 0; t < NT; ++t)
+    for (int t = 0; t < NT; ++t)
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj)
-          sh.S[sb][MT::trow(Lg, jj)][16 * t + Ln] = mkc<T>(Vr.get(t, jj), Vi.get(t, jj));
-      return;
-    }
-    auto body = [&](auto J) {
-#pragma unroll
-      for (int t = 0; t < NT; ++t) sh.S[sb][Lg][16 * t + Ln] = mkc<T>(Vr.get(t, J), Vi.get(t, J));
-    };
-    if constexpr (decltype(Vr)::bank) body(__builtin_amdgcn_readfirstlane(m / NT));
-    else with_j(m / NT, body);
+      for (int jj = 0; jj < 4; ++jj) sh.S[sb][MT::trow(Lg, jj)][16 * t + Ln] = mkc<T>(Vr.get(t, jj), Vi.get(t, jj));
   };
 
   // this wave's rows at the pivot columns q0..q3 -> Cr/Ci[wave] (the A operand
@@ -251,55 +212,30 @@ __device__ __forceinline__ void blk_solve(BlkShared<T, NT> &sh, const int lb, in
       }
     }
   };
-  // tile t += A W[wb] (a: tile row n, k = g).  FP64: four K=4 f64 MFMAs.
-  // FP32: the A operand split once per panel (prep), the B operand as
-  // published (Wb), four bf16 MFMAs per tile (bf_cmma).
-  constexpr bool kBf = BlkShared<T, NT>::BF, kS4 = BlkShared<T, NT>::S4;
-  typedef typename std::conditional<kBf, BfA, c2>::type AOp;
-  auto prep = [&](c2 a) -> AOp {
-    if constexpr (kBf) return bf_prep_a(a.x, a.y);
-    else return a;
-  };
-  auto mma_tile = [&](int t, const AOp &a, int wb) {
+  // tile t += A W[wb] (a: tile row n, k = g): four K=4 f64 MFMAs (complex 4M scheme)
+  auto mma_tile = [&](int t, const c2 &a, int wb) {
     v4 cr = Vr.tile(t), ci = Vi.tile(t);
-    if constexpr (kBf) {
-      bf_cmma(a, sh.Wb[wb][Lg][16 * t + Ln], cr, ci);  // B operand: k = g, column 16t + n
-    } else {
-      const c2 bb = sh.W[wb][Lg][16 * t + Ln];
-      cr = MT::mma(a.x, bb.x, cr);
-      cr = MT::mma(-a.y, bb.y, cr);
-      ci = MT::mma(a.x, bb.y, ci);
-      ci = MT::mma(a.y, bb.x, ci);
-    }
+    const c2 bb = sh.W[wb][Lg][16 * t + Ln];
+    cr = MT::mma(a.x, bb.x, cr);
+    cr = MT::mma(-a.y, bb.y, cr);
+    ci = MT::mma(a.x, bb.y, ci);
+    ci = MT::mma(a.y, bb.x, ci);
     Vr.set_tile(t, cr);
     Vi.set_tile(t, ci);
   };
   // the owner of panel kp, after factoring it: its tile += A1 W[w1] + A2 W[w2]
   // for panels kp-1 (pivots p, deferred) and kp (pivots q).  Its rows are as
-  // of panel kp-2 (it factored panel kp meanwhile): in its accumulators, and,
-  // when all sixteen are staged, in S[sb].  A1 = M[:, P], A2 = M[:, Q] + A1 W[w1][:, Q].
+  // of panel kp-2 (it factored panel kp meanwhile): in its accumulators and
+  // staged in S[sb].  A1 = M[:, P], A2 = M[:, Q] + A1 W[w1][:, Q].
   // (The panel's own rows of the tile are overwritten with their final values
   // afterwards.)
   auto update_tile2 = [&](int sb, int p0, int p1, int p2, int p3, int w1, int q0, int q1, int q2, int q3, int w2) {
     const int qg = Lg == 0 ? q0 : Lg == 1 ? q1 : Lg == 2 ? q2 : q3;
-    c2 a1, a2, cp[4];
-    if constexpr (kS4) {  // the FP32 accumulators are read by VGPR indexing
-      gather_cols(p0, p1, p2, p3);
-      wave_sync_lds();
-      a1 = mkc<T>(sh.Cr[wave][Ln][Lg], sh.Ci[wave][Ln][Lg]);
+    const int pg = Lg == 0 ? p0 : Lg == 1 ? p1 : Lg == 2 ? p2 : p3;
+    const c2 a1 = sh.S[sb][Ln][pg];
+    c2 a2 = sh.S[sb][Ln][qg], cp[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) cp[r] = mkc<T>(sh.Cr[wave][Ln][r], sh.Ci[wave][Ln][r]);
-      wave_sync_lds();
-      gather_cols(q0, q1, q2, q3);
-      wave_sync_lds();
-      a2 = mkc<T>(sh.Cr[wave][Ln][Lg], sh.Ci[wave][Ln][Lg]);
-    } else {
-      const int pg = Lg == 0 ? p0 : Lg == 1 ? p1 : Lg == 2 ? p2 : p3;
-      a1 = sh.S[sb][Ln][pg];
-      a2 = sh.S[sb][Ln][qg];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) cp[r] = sh.S[sb][Ln][r == 0 ? p0 : r == 1 ? p1 : r == 2 ? p2 : p3];
-    }
+    for (int r = 0; r < 4; ++r) cp[r] = sh.S[sb][Ln][r == 0 ? p0 : r == 1 ? p1 : r == 2 ? p2 : p3];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const c2 c = cp[r];
@@ -307,18 +243,17 @@ __device__ __forceinline__ void blk_solve(BlkShared<T, NT> &sh, const int lb, in
       a2.x = fma(c.x, wv.x, fma(-c.y, wv.y, a2.x));
       a2.y = fma(c.x, wv.y, fma(c.y, wv.x, a2.y));
     }
-    const AOp o1 = prep(a1), o2 = prep(a2);
 #pragma unroll
-    for (int t = 0; t < NT; ++t) mma_tile(t, o1, w1);
+    for (int t = 0; t < NT; ++t) mma_tile(t, a1, w1);
 #pragma unroll
-    for (int t = 0; t < NT; ++t) mma_tile(t, o2, w2);
+    for (int t = 0; t < NT; ++t) mma_tile(t, a2, w2);
     __builtin_amdgcn_wave_barrier();
   };
   // this wave's tile += tile[:, Q] W[wb]
   auto update_tile = [&](int q0, int q1, int q2, int q3, int wb) {
     gather_cols(q0, q1, q2, q3);
     wave_sync_lds();
-    const AOp a = prep(mkc<T>(sh.Cr[wave][Ln][Lg], sh.Ci[wave][Ln][Lg]));
+    const c2 a = mkc<T>(sh.Cr[wave][Ln][Lg], sh.Ci[wave][Ln][Lg]);
 #pragma unroll
     for (int t = 0; t < NT; ++t) mma_tile(t, a, wb);
     __builtin_amdgcn_wave_barrier();
@@ -332,7 +267,7 @@ __device__ __forceinline__ void blk_solve(BlkShared<T, NT> &sh, const int lb, in
     const int sb = kp1 % 3;
     if constexpr (!pre) wave_sync_lds();  // own staging just before (later: a barrier in between)
     // the panel's rows in the staged copy (row r: element j1 of lane group r)
-    auto prow = [&](int r) { return kS4 ? r : MT::trow(r, j1); };
+    auto prow = [&](int r) { return MT::trow(r, j1); };
     // the panel rows from the staged copy, one column per lane
     T pr[NU][4], pim[NU][4];
 #pragma unroll
@@ -471,7 +406,6 @@ __device__ __forceinline__ void blk_solve(BlkShared<T, NT> &sh, const int lb, in
         for (int r = 0; r < 4; ++r) {
           const T wx = pr[u][r] - (c == P[r] ? (T)1 : (T)0);
           sh.W[bo][r][Ll + 64 * u] = mkc<T>(wx, pim[u][r]);
-          if constexpr (kBf) sh.Wb[bo][r][Ll + 64 * u] = bf_pack_b(wx, pim[u][r]);
         }
       }
     }
@@ -547,131 +481,15 @@ __device__ __forceinline__ void blk_solve(BlkShared<T, NT> &sh, const int lb, in
   }
   __syncthreads();
 
-  if constexpr (REFINE) {
-    // ---- FP64 iterative refinement with the FP32 inverse --------------------
-    // the pivot columns of this lane's four rows (fixed from here on)
-    int pcs[4];
-#pragma unroll
-    for (int jj = 0; jj < 4; ++jj) pcs[jj] = sh.pcol_of_row[16 * wave + MT::trow(g, jj)];
-    int it = 0;
-    double last_ratio = 0.0;
-    bool converged_stop = false;
-    for (; it < max_ir; ++it) {
-      // r_i = 1 - sum_j Gamma[j, i] y_j - y_i / chi0_i, stored at column
-      // pcol_of_row[i] (where X consumes it); wave w: rows 4k + g for k = w, w + NT, ...
-      for (int ib = 4 * wave; ib < N; ib += 4 * NT) {
-        const int i = ib + g;
-        double2 acc = cz(0.0, 0.0);
-        if (i < N) {
-          // all NT loads of the row in flight at once (L2-resident Gamma); when
-          // every tile but the last lies inside the matrix (N > 16 (NT - 1), a
-          // wave-uniform test) only the last tile's load is guarded
-          const double2 *col = G + (size_t)N * i;
-          double2 gv[NT];
-          if (N > 16 * (NT - 1)) {
-#pragma unroll
-            for (int k = 0; k < NT - 1; ++k) gv[k] = col[n + 16 * k];
-            gv[NT - 1] = (n + 16 * (NT - 1) < N) ? col[n + 16 * (NT - 1)] : cz(0.0, 0.0);
-          } else {
-#pragma unroll
-            for (int k = 0; k < NT; ++k) gv[k] = (n + 16 * k < N) ? col[n + 16 * k] : cz(0.0, 0.0);
-          }
-#pragma unroll
-          for (int k = 0; k < NT; ++k) {  // acc += gv[k] y[n + 16k], as four FMAs
-            const double2 yk = sh.y[n + 16 * k];
-            acc.x = fma(gv[k].x, yk.x, fma(-gv[k].y, yk.y, acc.x));
-            acc.y = fma(gv[k].x, yk.y, fma(gv[k].y, yk.x, acc.y));
-          }
-        }
-        acc.x = row_sum16(acc.x);
-        acc.y = row_sum16(acc.y);
-        if (n == 0 && i < N) {
-          const double2 yi = sh.y[i];
-          const double2 d = cmul(yi, sh.dinv[i]);
-          sh.res[sh.pcol_of_row[pos_of_row(i)]] = mkc<T>((T)(1.0 - acc.x - d.x), (T)(-acc.y - d.y));
-        }
-      }
-      if (tid < NC - N) sh.res[N + tid] = mkc<T>(0, 0);
-      __syncthreads();
-      // dy[p_r] = sum_c X[r][c] r[row_of(c)], r[row_of(c)] stored at c  (FP32 products)
-      float dmax = 0.0f, ymax = 0.0f;
-#pragma unroll
-      for (int jj = 0; jj < 4; ++jj) {
-        T sx = 0, sy = 0;
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          const c2 rv = sh.res[16 * t + n];
-          const T rx = rv.x, ry = rv.y;
-          sx = fma(Vr.get(t, jj), rx, fma(-Vi.get(t, jj), ry, sx));
-          sy = fma(Vr.get(t, jj), ry, fma(Vi.get(t, jj), rx, sy));
-        }
-        sx = row_sum16(sx);
-        sy = row_sum16(sy);
-        const int pc = pcs[jj];
-        if (n == 0 && pc < N) {
-          const double2 yo = sh.y[pc];
-          const double2 yn = cz(yo.x + (double)sx, yo.y + (double)sy);
-          sh.y[pc] = yn;
-          dmax = fmaxf(dmax, fabsf(sx) + fabsf(sy));
-          ymax = fmaxf(ymax, (float)(fabs(yn.x) + fabs(yn.y)));
-        }
-      }
-      // wave maxima by DPP (non-negative floats order like their bit patterns;
-      // a NaN stays the maximum), then across the waves through LDS
-      const unsigned dk = wave_max_u32(__float_as_uint(dmax)), yk = wave_max_u32(__float_as_uint(ymax));
-      if (lane == 63) {
-        sh.redf[it & 1][0][wave] = __uint_as_float(dk);
-        sh.redf[it & 1][1][wave] = __uint_as_float(yk);
-      }
-      __syncthreads();  // redf[it & 1] is rewritten two steps later
-      float dm = 0.0f, ym = 0.0f;
-#pragma unroll
-      for (int k = 0; k < NT; ++k) {
-        dm = fmaxf(dm, sh.redf[it & 1][0][k]);
-        ym = fmaxf(ym, sh.redf[it & 1][1][k]);
-      }
-      const double ratio = (double)dm / (double)ym;
-      // Stop when converged: the correction is at FP64 rounding, or the next
-      // one, extrapolated from the observed contraction ratio / last_ratio (the
-      // error left in y), is below 2^-48 |y|; or when stagnating at the FP64
-      // residual's own rounding level (kappa(A) * 2^-53: the correction no
-      // longer halves).
-      const bool converged = !(ratio > 0x1p-50) || (it > 0 && ratio * ratio <= 0x1p-48 * last_ratio);
-      const bool stagnant = it > 0 && !(ratio < 0.5 * last_ratio);
-      const bool stop = converged || stagnant;
-      last_ratio = ratio;
-      if (stop) {
-        converged_stop = converged;
-        ++it;
-        break;
-      }
-    }
-    if (tid == 0) {
-      atomicMax(&counters[1], (unsigned long long)it);
-      atomicAdd(&counters[5], (unsigned long long)it);
-      // a refinement that did not converge and whose last correction was above
-      // 2^-30 (or that met a zero FP32 pivot) is listed for the FP64 re-solve
-      // when the caller provides the list (room for every matrix)
-      if ((!converged_stop && !(last_ratio <= 0x1p-30)) || sh.zero_pivot) {
-        const unsigned long long k = atomicAdd(&counters[2], 1ull);
-        if (redo) redo[k] = lb;
-      }
-      unsigned long long bits;
-      const double lr = last_ratio == last_ratio ? last_ratio : 1.0;
-      memcpy(&bits, &lr, 8);
-      atomicMax(&counters[3], bits);  // non-negative doubles order like their bit patterns
-    }
-  }
-
   // ---- chi and gamma ---------------------------------------------------------
   double part = 0.0;
   for (int k = tid; k < N; k += 64 * NT) part += sh.y[k].x;
   for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
-  if (lane == 0) sh.red[0][wave] = part;
+  if (lane == 0) sh.red[wave] = part;
   __syncthreads();
   double tot = 0.0;
 #pragma unroll
-  for (int k = 0; k < NT; ++k) tot += sh.red[0][k];
+  for (int k = 0; k < NT; ++k) tot += sh.red[k];
   const double chi = tot / (beta * beta);
   const double fac = 1.0 + s * U * chi;
   for (int k = tid; k < N; k += 64 * NT) {
@@ -680,31 +498,29 @@ __device__ __forceinline__ void blk_solve(BlkShared<T, NT> &sh, const int lb, in
   }
   if (tid == 0) {
     chi_out[q + (size_t)Nq * w] = chi;
-    // a listed matrix's zero pivot is decided by its FP64 re-solve
-    if (sh.zero_pivot && !(REFINE && redo)) atomicAdd(&counters[0], 1ull);
+    if (sh.zero_pivot) atomicAdd(&counters[0], 1ull);
   }
 }
 
 // every (q, w), one workgroup each, XCD-aware
-template <typename T, int NT, bool REFINE>
-__global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_kernel(int N, int Nq, int Nc, int shell,
+template <typename T, int NT>
+__global__ __launch_bounds__(64 * NT, 1) void bse_blk_kernel(int N, int Nq, int Nc, int shell,
                                                           const double2 *__restrict__ Gam,
                                                           const double2 *__restrict__ chi0, double s, double U,
-                                                          double beta, int max_ir, double *__restrict__ chi_out,
+                                                          double beta, double *__restrict__ chi_out,
                                                           double2 *__restrict__ gamma_out,
-                                                          unsigned long long *__restrict__ counters,
-                                                          int *__restrict__ redo) {
+                                                          unsigned long long *__restrict__ counters) {
   __shared__ BlkShared<T, NT> sh;
-  blk_solve<T, NT, REFINE>(sh, xcd_logical_block(blockIdx.x, gridDim.x), N, Nq, Nc, shell, Gam, chi0, s, U, beta,
-                           max_ir, chi_out, gamma_out, counters, redo);
+  blk_solve<T, NT>(sh, xcd_logical_block(blockIdx.x, gridDim.x), N, Nq, Nc, shell, Gam, chi0, s, U, beta, chi_out,
+                   gamma_out, counters);
 }
 
-// FP64 over the matrices listed in list[0 .. *count - 1] (q + Nq w each): the
+// over the matrices listed in list[0 .. *count - 1] (q + Nq w each): the
 // re-solve of a mixed call's refinement failures.  The count is read on the
 // device, so the host never waits for it; each workgroup takes the items
 // blockIdx.x, blockIdx.x + gridDim.x, ...
 template <typename T, int NT>
-__global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_list_kernel(
+__global__ __launch_bounds__(64 * NT, 1) void bse_blk_list_kernel(
     int N, int Nq, int Nc, int shell, const double2 *__restrict__ Gam, const double2 *__restrict__ chi0, double s,
     double U, double beta, double *__restrict__ chi_out, double2 *__restrict__ gamma_out,
     unsigned long long *__restrict__ counters, const int *__restrict__ list,
@@ -712,32 +528,28 @@ __global__ __launch_bounds__(64 * NT, (blk_min_waves<T, NT>())) void bse_blk_lis
   __shared__ BlkShared<T, NT> sh;
   const int nitems = __builtin_amdgcn_readfirstlane((int)*count);
   for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
-    blk_solve<T, NT, false>(sh, list[item], N, Nq, Nc, shell, Gam, chi0, s, U, beta, 0, chi_out, gamma_out, counters,
-                            nullptr);
+    blk_solve<T, NT>(sh, list[item], N, Nq, Nc, shell, Gam, chi0, s, U, beta, chi_out, gamma_out, counters);
     __syncthreads();  // the shared state is reused by the next item
   }
 }
 
-template <typename T, int NT, bool REFINE>
+template <int NT>
 int run_blk(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam, const double2 *chi0,
-            double s, double U, double beta, int max_ir, double *chi, double2 *gamma, unsigned long long *cnt,
-            int *redo, const int *list, const unsigned long long *count, int max_items, hipStream_t st) {
+            double s, double U, double beta, double *chi, double2 *gamma, unsigned long long *cnt, const int *list,
+            const unsigned long long *count, int max_items, hipStream_t st) {
   if (N > 16 * NT) return ldga_fail(ctx, LDGA_ERR_SHAPE, "BSE tile too small for N=%d", N);
   if (list) {
-    if constexpr (!REFINE) {
-      if (max_items <= 0) return LDGA_OK;
-      const unsigned grid = (unsigned)std::min(max_items, 1024);
-      bse_blk_list_kernel<T, NT><<<dim3(grid), dim3(64 * NT), 0, st>>>(N, Nq, Nc, shell, Gam, chi0, s, U, beta, chi,
-                                                                        gamma, cnt, list, count);
-      LDGA_HIP_TRY(ctx, hipGetLastError());
-      return LDGA_OK;
-    }
-    return ldga_fail(ctx, LDGA_ERR_ARG, "a matrix list is solved in FP64 only");
+    if (max_items <= 0) return LDGA_OK;
+    const unsigned grid = (unsigned)std::min(max_items, 1024);
+    bse_blk_list_kernel<double, NT><<<dim3(grid), dim3(64 * NT), 0, st>>>(N, Nq, Nc, shell, Gam, chi0, s, U, beta,
+                                                                          chi, gamma, cnt, list, count);
+    LDGA_HIP_TRY(ctx, hipGetLastError());
+    return LDGA_OK;
   }
   const unsigned nblocks = (unsigned)Nq * Nw;
   if (nblocks == 0) return LDGA_OK;
-  bse_blk_kernel<T, NT, REFINE><<<dim3(nblocks), dim3(64 * NT), 0, st>>>(N, Nq, Nc, shell, Gam, chi0, s, U, beta,
-                                                                         max_ir, chi, gamma, cnt, redo);
+  bse_blk_kernel<double, NT><<<dim3(nblocks), dim3(64 * NT), 0, st>>>(N, Nq, Nc, shell, Gam, chi0, s, U, beta, chi,
+                                                                      gamma, cnt);
   LDGA_HIP_TRY(ctx, hipGetLastError());
   return LDGA_OK;
 }
@@ -752,9 +564,8 @@ int launch_bse_blk_f64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, 
                        unsigned long long *cnt, const int *list, const unsigned long long *count, int max_items,
                        hipStream_t st) {
   const int nt = (N + 15) / 16;
-#define LDGA_BLK_F64(NT_)                                                                                     \
-  return run_blk<double, NT_, false>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, 0, chi, gamma, cnt, nullptr, \
-                                     list, count, max_items, st)
+#define LDGA_BLK_F64(NT_) \
+  return run_blk<NT_>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, chi, gamma, cnt, list, count, max_items, st)
   switch (nt) {
     case 1:
     case 2:
@@ -767,37 +578,4 @@ int launch_bse_blk_f64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, 
     default: return ldga_fail(ctx, LDGA_ERR_SHAPE, "FP64 blocked BSE supports Nnu <= 128 (got %d)", N);
   }
 #undef LDGA_BLK_F64
-}
-
-// Mixed: FP32 inverse on MFMA + FP64 refinement, Nnu <= 160.  With redo !=
-// NULL (room for Nq Nw ints) the matrices whose refinement failed are listed
-// there; counters[2] holds their number.
-int launch_bse_mixed_bgj(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam,
-                         const double2 *chi0, double s, double U, double beta, double *chi, double2 *gamma,
-                         unsigned long long *cnt, int *redo, hipStream_t st);
-
-int launch_bse_mixed(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell, const double2 *Gam, const double2 *chi0,
-                     double s, double U, double beta, double *chi, double2 *gamma, unsigned long long *cnt,
-                     int *redo, hipStream_t st) {
-  const char *e = getenv("LDGA_BSE_MIXED");
-  if (!(e && strcmp(e, "panel") == 0))
-    return launch_bse_mixed_bgj(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, chi, gamma, cnt, redo, st);
-  const int nt = (N + 15) / 16, ir = 6;
-#define LDGA_BLK_MIXED(NT_)                                                                                   \
-  return run_blk<float, NT_, true>(ctx, N, Nq, Nw, Nc, shell, Gam, chi0, s, U, beta, ir, chi, gamma, cnt, redo, \
-                                   nullptr, nullptr, 0, st)
-  switch (nt) {
-    case 1:
-    case 2:
-    case 3: LDGA_BLK_MIXED(3);
-    case 4: LDGA_BLK_MIXED(4);
-    case 5: LDGA_BLK_MIXED(5);
-    case 6: LDGA_BLK_MIXED(6);
-    case 7: LDGA_BLK_MIXED(7);
-    case 8: LDGA_BLK_MIXED(8);
-    case 9: LDGA_BLK_MIXED(9);
-    case 10: LDGA_BLK_MIXED(10);
-    default: return ldga_fail(ctx, LDGA_ERR_SHAPE, "mixed BSE supports Nnu <= 160 (got %d)", N);
-  }
-#undef LDGA_BLK_MIXED
 }

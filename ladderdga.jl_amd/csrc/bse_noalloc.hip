@@ -54,5 +54,5 @@ extern "C" int ldga_chi_gamma_noalloc_dev(ldga_ctx *ctx, int s, int precision, c
   unsigned long long *cnt = reinterpret_cast<unsigned long long *>(base);
   int *redo = reinterpret_cast<int *>(base + kNoallocCounterBytes);
   return launch_bse_with(ctx, s, precision, fr, U, beta, (const double2 *)Gamma, (const double2 *)chi0, (double *)chi,
-                         (double2 *)gamma, report, ldga_stream(ctx, stream), cnt, redo, false);
+                         (double2 *)gamma, report, ldga_stream(ctx, stream), cnt, redo);
 }

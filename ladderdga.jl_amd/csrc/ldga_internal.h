@@ -116,7 +116,7 @@ int launch_bse(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double 
                const double2 *chi0, double *chi, double2 *gamma, ldga_bse_report *rep, hipStream_t st);
 int launch_bse_with(ldga_ctx *ctx, int s, int precision, const ldga_freq *fr, double U, double beta,
                     const double2 *Gam, const double2 *chi0, double *chi, double2 *gamma, ldga_bse_report *rep,
-                    hipStream_t st, unsigned long long *cnt, int *redo, bool register_gj);
+                    hipStream_t st, unsigned long long *cnt, int *redo);
 int launch_lambda0(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double2 *F, const double2 *chi0,
                    double2 *lambda0, hipStream_t st);
 int launch_sum_kw(ldga_ctx *ctx, const double *chi, int Nw, int w_lo, int w_hi, double c2, double beta, double lam,

@@ -122,18 +122,6 @@ def test_bse_mixed(ctx, name, kw, ch):
     assert relerr(γ.data, rγ) <= TOL
 
 
-def test_bse_register_gj_kernel(ctx, monkeypatch):
-    """The second FP64 implementation (register-tile Gauss-Jordan, LDGA_BSE_F64=gj) agrees too."""
-    import ladderdga_hip as L
-    inp, oK, omP, osP = setup("2d16")
-    ref_χ0 = O.calc_bubble(inp.G_fft, inp.G_rfft, oK, omP, osP)
-    χ0 = L.χ0T(np.asfortranarray(ref_χ0), inp.mP.β)
-    monkeypatch.setenv("LDGA_BSE_F64", "gj")
-    χ, γ = L.calc_χγ("m", inp.Γ_m, χ0, inp.kG, inp.mP, inp.sP, ctx=ctx)
-    monkeypatch.delenv("LDGA_BSE_F64")
-    rχ, rγ = O.calc_χγ("m", inp.Γ_m, ref_χ0, oK, omP, osP)
-    assert relerr(χ.data, rχ) <= TOL and relerr(γ.data, rγ) <= TOL
-
 
 def test_bse_mixed_nnu160(ctx):
     """Nν = 160 (the 32^3 workload's matrix size) on a small 3D grid: mixed precision (FP32 MFMA inverse)
