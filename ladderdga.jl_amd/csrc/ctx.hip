@@ -121,8 +121,8 @@ int ldga_create(int device, ldga_ctx **out) {
 }
 
 static void free_kgrid(ldga_ctx *ctx) {
-  const void *p[] = {ctx->oct.oct2wedge, ctx->oct.wedge2oct, ctx->oct.octfull, ctx->oct.octw,  ctx->oct.octphase,
-                     ctx->oct.octpw,     ctx->oct.cosT,      ctx->oct.kmult,   ctx->oct.eps};
+  const void *p[] = {ctx->oct.wedgecoord, ctx->oct.wedgefull, ctx->oct.wedgew, ctx->oct.wedgephase,
+                     ctx->oct.wedgepw,    ctx->oct.cosT,      ctx->oct.kmult,  ctx->oct.eps};
   for (const void *q : p)
     if (q) hipFree(const_cast<void *>(q));
   ctx->oct = OctTables{};
@@ -251,28 +251,31 @@ int ldga_set_kgrid(ldga_ctx *ctx, int D, int Ns, int Nq, const int32_t *wedge, c
     if (idx.count(key)) return ldga_fail(ctx, LDGA_ERR_SHAPE, "wedge point %d repeated", q);
     idx[key] = q;
   }
-  std::vector<int32_t> o2w(NO), w2o(Nq), ofull(NO);
-  std::vector<double> ow(NO), cosT(Ns), kmult(Nq, 0.0);
-  std::vector<double2> oph(NO), opw(NO);
-  const int Nk = D == 2 ? Ns * Ns : Ns * Ns * Ns;
+  // every octant point is a permutation of exactly one wedge point
   for (int o = 0; o < NO; ++o) {
     int c[3] = {o % n, (o / n) % n, D == 3 ? o / (n * n) : 0};
-    o2w[o] = idx.at(canon_key(c, D, n));
-    ofull[o] = c[0] + Ns * c[1] + (D == 3 ? Ns * Ns * c[2] : 0);
+    if (!idx.count(canon_key(c, D, n)))
+      return ldga_fail(ctx, LDGA_ERR_SHAPE, "octant point %d has no wedge point", o);
+  }
+  std::vector<int32_t> wcoord(Nq), wfull(Nq);
+  std::vector<double> ww(Nq), cosT(Ns), kmult(Nq, 0.0);
+  std::vector<double2> wph(Nq), wpw(Nq);
+  const int Nk = D == 2 ? Ns * Ns : Ns * Ns * Ns;
+  for (int q = 0; q < Nq; ++q) {
+    const int *c = &wedge[D * q];
+    const int c2 = D == 3 ? c[2] : 0;
+    wcoord[q] = c[0] | (c[1] << 8) | (c2 << 16);
+    wfull[q] = c[0] + Ns * c[1] + Ns * Ns * c2;
     double w = 1.0;
     int sum = 0;
     for (int d = 0; d < D; ++d) {
       w *= (c[d] == 0 || c[d] == Ns / 2) ? 1.0 : 2.0;
       sum += c[d];
     }
-    ow[o] = w;
+    ww[q] = w;
     const double ang = -2.0 * M_PI * (double)(sum % Ns) / (double)Ns;
-    oph[o] = make_double2(std::cos(ang), std::sin(ang));
-    opw[o] = make_double2(w * oph[o].x, w * oph[o].y);
-  }
-  for (int q = 0; q < Nq; ++q) {
-    const int *c = &wedge[D * q];
-    w2o[q] = c[0] + n * c[1] + (D == 3 ? n * n * c[2] : 0);
+    wph[q] = make_double2(std::cos(ang), std::sin(ang));
+    wpw[q] = make_double2(w * wph[q].x, w * wph[q].y);
   }
   for (int m = 0; m < Ns; ++m) cosT[m] = std::cos(2.0 * M_PI * (double)m / (double)Ns);
   {
@@ -296,18 +299,16 @@ int ldga_set_kgrid(ldga_ctx *ctx, int D, int Ns, int Nq, const int32_t *wedge, c
   };
   int rc = 0;
   void *p;
-  if ((rc = up(o2w.data(), NO * 4, &p))) return rc;
-  T.oct2wedge = (int32_t *)p;
-  if ((rc = up(w2o.data(), Nq * 4, &p))) return rc;
-  T.wedge2oct = (int32_t *)p;
-  if ((rc = up(ofull.data(), NO * 4, &p))) return rc;
-  T.octfull = (int32_t *)p;
-  if ((rc = up(ow.data(), NO * 8, &p))) return rc;
-  T.octw = (double *)p;
-  if ((rc = up(oph.data(), NO * 16, &p))) return rc;
-  T.octphase = (double2 *)p;
-  if ((rc = up(opw.data(), NO * 16, &p))) return rc;
-  T.octpw = (double2 *)p;
+  if ((rc = up(wcoord.data(), Nq * 4, &p))) return rc;
+  T.wedgecoord = (int32_t *)p;
+  if ((rc = up(wfull.data(), Nq * 4, &p))) return rc;
+  T.wedgefull = (int32_t *)p;
+  if ((rc = up(ww.data(), Nq * 8, &p))) return rc;
+  T.wedgew = (double *)p;
+  if ((rc = up(wph.data(), Nq * 16, &p))) return rc;
+  T.wedgephase = (double2 *)p;
+  if ((rc = up(wpw.data(), Nq * 16, &p))) return rc;
+  T.wedgepw = (double2 *)p;
   if ((rc = up(cosT.data(), Ns * 8, &p))) return rc;
   T.cosT = (double *)p;
   if ((rc = up(kmult.data(), Nq * 8, &p))) return rc;

@@ -10,8 +10,13 @@
 // are even in every axis, so the sum folds onto the octant R in [0,Ns/2]^D
 // with multiplicity w(R) and a separable cosine kernel: one small dense
 // "mode product" per axis, evaluated in LDS; the wedge points are then read
-// from the transformed octant.
+// from the transformed octant.  The functions are also symmetric under
+// permutations of the axes (they depend on k through the dispersion of the
+// square / simple-cubic lattice), so every product and sum over the octant is
+// formed on the wedge points only and written to the permuted copies (the
+// orbit) of each wedge point in LDS.
 #include <algorithm>
+#include <type_traits>
 
 #include "ldga_internal.h"
 #include "ldga_reduce.h"
@@ -32,8 +37,8 @@ constexpr int kNT = 256;  // threads per octant workgroup
 // (error below 2e-14 for Ns <= 64).  Lines go in rounds of kNT / nqb lines; a
 // round writes only its own lines, so one barrier per round separates its
 // reads from its writes.
-constexpr int kQ = 4;
-__device__ __forceinline__ void oct_pass(double2 *buf, const double *cs, int ax, const OctTables &T) {
+constexpr int kQ = 3;
+__device__ __forceinline__ void oct_pass_paired(double2 *buf, const double *cs, int ax, const OctTables &T) {
   const int n = T.n, half = T.Ns / 2, stride = T.stride[ax];
   const int np = T.Ns / 4 + 1;  // q = 0 .. Ns/4 with their partners Ns/2 - q cover 0 .. Ns/2
   const int nqb = (np + kQ - 1) / kQ, nlines = T.NO / n, per_round = kNT / nqb;
@@ -86,89 +91,303 @@ __device__ __forceinline__ void oct_pass(double2 *buf, const double *cs, int ax,
   __syncthreads();
 }
 
-__device__ __forceinline__ int trunc_half(int x) { return x / 2; }  // C division truncates toward 0 like Julia trunc
-
-// Wedge points per thread (q = threadIdx.x + k kNT): Nq <= 969 for the grids
-// ldga_set_kgrid accepts (2D Ns <= 64: 561, 3D Ns <= 32: 969).
-constexpr int kMaxQ = 4;
-
-// This thread's octant points o = threadIdx.x + p kNT, p < MAXPT: their index
-// in the full grid and octphase * octw, loaded once with every load in flight
-// together (the points past NO get index 0 and weight 0).
-template <int MAXPT>
-struct OctPts {
-  int f[MAXPT];
-  double2 pw[MAXPT];
-  __device__ __forceinline__ void load(const OctTables &T) {
+// The same mode product for Ns divisible by 4 (h = Ns/2 = 2g), with one more
+// split.  Folding the line, u[r] = x[r] + x[h-r] and v[r] = x[r] - x[h-r]
+// (r < g; u[g] = x[g]), gives
+//   y[2m]   = sum_{r<=g} u[r] cos(pi 2m r / h)        (cos is even about r = g)
+//   y[2m+1] = sum_{r<g}  v[r] cos(pi (2m+1) r / h)    (odd about r = g, zero at g)
+// and each half keeps the E/O pairing: outputs 2m and h-2m from u, outputs 2m+1
+// and h-2m-1 from v.  The lines are half as long: half the multiply-adds of
+// oct_pass_paired.  The fold is done in place (u at r, v at h-r) by the
+// threads of the line, one barrier before the sums.
+__device__ __forceinline__ void oct_pass_fold(double2 *buf, const double *cs, int ax, const OctTables &T) {
+  const int n = T.n, h = T.Ns / 2, g = h / 2, stride = T.stride[ax];
+  const int ne = g / 2 + 1, nod = (g + 1) / 2;  // pairs from u, pairs from v
+  const int nbe = (ne + kQ - 1) / kQ, nqb = nbe + (nod + kQ - 1) / kQ;
+  const int nlines = T.NO / n, per_round = kNT / nqb;
+  const int li = threadIdx.x / nqb, b = threadIdx.x - li * nqb;
+  const bool odd = b >= nbe;
+  const int m0 = (odd ? b - nbe : b) * kQ, nm = odd ? nod : ne;
+  const int cnt = odd ? g : g + 1;             // points of the folded line
+  const int rstep = odd ? -stride : stride;    // v[r] sits at h - r
+  double tw[kQ];
 #pragma unroll
-    for (int p = 0; p < MAXPT; ++p) {
-      const int o = threadIdx.x + p * kNT;
-      const bool v = o < T.NO;
-      f[p] = v ? T.octfull[o] : 0;
-      pw[p] = v ? T.octpw[o] : cz(0.0, 0.0);
+  for (int j = 0; j < kQ; ++j) {
+    const int m = m0 + j;
+    tw[j] = m < nm ? 2.0 * cs[odd ? 2 * m + 1 : 2 * m] : 0.0;
+  }
+  for (int l0 = 0; l0 < nlines; l0 += per_round) {
+    const int line = l0 + li;
+    const bool active = li < per_round && line < nlines;
+    const int base = (line % stride) + (line / stride) * stride * n;
+    if (active) {
+      for (int r = b; r < g; r += nqb) {
+        const double2 x0 = buf[base + r * stride], x1 = buf[base + (h - r) * stride];
+        buf[base + r * stride] = cadd(x0, x1);
+        buf[base + (h - r) * stride] = csub(x0, x1);
+      }
     }
+    __syncthreads();
+    double2 E[kQ], O[kQ];
+#pragma unroll
+    for (int j = 0; j < kQ; ++j) E[j] = O[j] = cz(0.0, 0.0);
+    if (active) {
+      double c[kQ], cm[kQ];
+#pragma unroll
+      for (int j = 0; j < kQ; ++j) {
+        c[j] = 1.0;
+        cm[j] = 0.5 * tw[j];
+      }
+      auto step = [&](double2 xe, double2 xo) {
+#pragma unroll
+        for (int j = 0; j < kQ; ++j) {
+          E[j].x = fma(c[j], xe.x, E[j].x);
+          E[j].y = fma(c[j], xe.y, E[j].y);
+          const double c1 = fma(tw[j], c[j], -cm[j]);
+          O[j].x = fma(c1, xo.x, O[j].x);
+          O[j].y = fma(c1, xo.y, O[j].y);
+          cm[j] = c1;
+          c[j] = fma(tw[j], c1, -c[j]);
+        }
+      };
+      // nit steps of two points, the same count for both kinds; the reads of
+      // the last step stay inside the line (points up to g + 1 <= h from either
+      // end) and its points past cnt are zeroed
+      const int nit = (g + 2) / 2;
+      int pos = base + (odd ? h * stride : 0);
+      for (int it = 0; it + 1 < nit; ++it, pos += 2 * rstep) step(buf[pos], buf[pos + rstep]);
+      const int r = 2 * (nit - 1);
+      double2 xe = buf[pos], xo = buf[pos + rstep];
+      if (r >= cnt) xe = cz(0.0, 0.0);
+      if (r + 1 >= cnt) xo = cz(0.0, 0.0);
+      step(xe, xo);
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int j = 0; j < kQ; ++j) {
+        const int m = m0 + j;
+        if (m < nm) {
+          const int q = odd ? 2 * m + 1 : 2 * m;
+          buf[base + q * stride] = cadd(E[j], O[j]);
+          if (h - q != q) buf[base + (h - q) * stride] = csub(E[j], O[j]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void oct_pass(double2 *buf, const double *cs, int ax, const OctTables &T) {
+  if ((T.Ns & 3) == 0)
+    oct_pass_fold(buf, cs, ax, T);
+  else
+    oct_pass_paired(buf, cs, ax, T);
+}
+
+// The passes on the vector ALU (any even Ns).
+struct ValuPass {
+  __device__ __forceinline__ void init(const OctTables &) {}
+  __device__ __forceinline__ void run(double2 *buf, const double *cs, int ax, const OctTables &T) const {
+    oct_pass(buf, cs, ax, T);
   }
 };
 
-// the octant index of this thread's wedge points (index 0 past Nq), for the
-// final reads of the transformed octant
-__device__ __forceinline__ void load_w2o(const OctTables &T, int (&w2o)[kMaxQ]) {
+// sum of v over the four 16-lane rows of the wave, in every lane (permlane swaps)
+__device__ __forceinline__ double row_sum4(double v) {
+  const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+  const auto l32 = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto h32 = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  const double y = __hiloint2double(h32[0], l32[0]) + __hiloint2double(h32[1], l32[1]);
+  const unsigned ylo = __double2loint(y), yhi = __double2hiint(y);
+  const auto l16 = __builtin_amdgcn_permlane16_swap(ylo, ylo, false, false);
+  const auto h16 = __builtin_amdgcn_permlane16_swap(yhi, yhi, false, false);
+  return __hiloint2double(h16[0], l16[0]) + __hiloint2double(h16[1], l16[1]);
+}
+
+// The folded pass on the FP64 matrix cores, for Ns = 32, 48, 64 (g = Ns/4 a
+// multiple of 4, at most 16).  A wave takes 8 lines (16 real columns: the
+// real and imaginary parts) at a time.  With u and v of the fold,
+//   y[2m]   = sum_{r<g} C1[m][r] u[r] + (-1)^m x[g]     C1[m][r] = cos(pi 2m r / h)
+//   y[2m+1] = sum_{r<g} C2[m][r] v[r]                   C2[m][r] = cos(pi (2m+1) r / h)
+//   y[h]    = sum_{r<g} (-1)^r u[r] + (-1)^g x[g]
+// for m < g: two 16 x 16 x g products (v_mfma_f64_16x16x4f64, g/4 steps each),
+// the cosine tiles held in registers for the whole kernel (A operand: lane l,
+// row l % 16, k = l / 16), the fold done as the B operand is read (column l %
+// 16, k = l / 16), and y[h] as a lane sum over the four rows.  Accumulator
+// element e of lane l is row l / 16 + 4 e, column l % 16.  A wave reads and
+// writes only its own lines, so a pass needs one barrier, at its end.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+struct MfmaPass {
+  double ae[4], ao[4];
+  __device__ __forceinline__ void init(const OctTables &T) {
+    const int lane = threadIdx.x & 63, i = lane & 15, k = lane >> 4, Ns = T.Ns, g = Ns / 4;
 #pragma unroll
-  for (int k = 0; k < kMaxQ; ++k) {
-    const int q = threadIdx.x + k * kNT;
-    w2o[k] = q < T.Nq ? T.wedge2oct[q] : 0;
+    for (int kb = 0; kb < 4; ++kb) {
+      const int r = 4 * kb + k;
+      const bool v = i < g && r < g;
+      ae[kb] = v ? T.cosT[(2 * i * r) % Ns] : 0.0;
+      ao[kb] = v ? T.cosT[((2 * i + 1) * r) % Ns] : 0.0;
+    }
+  }
+  __device__ __forceinline__ void run(double2 *buf, const double *, int ax, const OctTables &T) const {
+    const int n = T.n, h = T.Ns / 2, g = h / 2, nkb = g / 4, stride = T.stride[ax], nlines = T.NO / n;
+    const int nct = (nlines + 7) / 8;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = lane & 15, k = lane >> 4, part = j & 1;
+    double *bd = reinterpret_cast<double *>(buf);
+    // the extra tile of a pass goes to a different wave in each pass and workgroup
+    const int rot = (blockIdx.x + ax) & (kNT / 64 - 1);
+    for (int ct = (wave + kNT / 64 - rot) & (kNT / 64 - 1); ct < nct; ct += kNT / 64) {
+      const int line = ct * 8 + (j >> 1);
+      const bool lv = line < nlines;
+      const int lc = lv ? line : nlines - 1;  // past the last line: read a valid one, write nothing
+      int base;                               // octant index of the line's point 0
+      if (ax == 0)
+        base = lc * n;
+      else if (ax == T.D - 1)
+        base = lc;
+      else {  // 3D, axis 1: line = a + n c
+        const int c = (int)(((float)lc + 0.5f) / (float)n);
+        base = lc + c * (n * n - n);
+      }
+      const int step = 8 * stride;  // doubles per 4 points
+      const int plo = 2 * (base + k * stride) + part, phi = 2 * (base + (h - k) * stride) + part;
+      dbl4 de = {0.0, 0.0, 0.0, 0.0}, dd = {0.0, 0.0, 0.0, 0.0};
+      double s = 0.0;
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb) {
+        if (kb < nkb) {
+          const double x0 = bd[plo + kb * step], x1 = bd[phi - kb * step];
+          const double u = x0 + x1, v = x0 - x1;
+          de = __builtin_amdgcn_mfma_f64_16x16x4f64(ae[kb], u, de, 0, 0, 0);
+          dd = __builtin_amdgcn_mfma_f64_16x16x4f64(ao[kb], v, dd, 0, 0, 0);
+          s += u;
+        }
+      }
+      const double xg = bd[2 * (base + g * stride) + part];
+      const double xs = (k & 1) ? -xg : xg;  // (-1)^m x[g] for the rows m = k + 4e
+      s = row_sum4((k & 1) ? -s : s);       // (-1)^r = (-1)^k for r = 4 kb + k
+      if (lv) {
+        const int w = 2 * (base + 2 * k * stride) + part, wstep = 16 * stride;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          if (4 * e < g) {
+            bd[w + e * wstep] = de[e] + xs;
+            bd[w + 2 * stride + e * wstep] = dd[e];
+          }
+        }
+        if (k == 0) bd[2 * (base + h * stride) + part] = s + ((g & 1) ? -xg : xg);
+      }
+    }
+    __syncthreads();
+  }
+};
+
+// the matrix-core pass applies to the grid
+__host__ __device__ inline bool mfma_pass_fits(int Ns) { return Ns == 32 || Ns == 48 || Ns == 64; }
+
+__device__ __forceinline__ int trunc_half(int x) { return x / 2; }  // C division truncates toward 0 like Julia trunc
+
+// Wedge points per thread (q = threadIdx.x + k kNT, k < NQS): NQS = Nq / kNT
+// rounded up, at most 4 (Nq <= 969 for the grids ldga_set_kgrid accepts: 2D
+// Ns <= 64: 561, 3D Ns <= 32: 969).  The kernels are instantiated for NQS = 1..4.
+
+// This thread's wedge points: packed coordinates (-1 past Nq) and the octant
+// index of the point itself.
+template <int NQS>
+struct WedgeSlots {
+  int pc[NQS];
+  __device__ __forceinline__ void load(const OctTables &T) {
+#pragma unroll
+    for (int k = 0; k < NQS; ++k) {
+      const int q = threadIdx.x + k * kNT;
+      pc[k] = q < T.Nq ? T.wedgecoord[q] : -1;
+    }
+  }
+  __device__ __forceinline__ bool valid(int k) const { return pc[k] >= 0; }
+  __device__ __forceinline__ int oct(int k, const OctTables &T) const {
+    return (pc[k] & 0xff) + T.n * (((pc[k] >> 8) & 0xff) + T.n * (pc[k] >> 16));
+  }
+};
+
+// v stored at every octant point of the orbit of packed coordinates pc (a
+// repeated coordinate writes one point twice, with the same value)
+__device__ __forceinline__ void orbit_store(double2 *buf, const OctTables &T, int pc, double2 v) {
+  const int a = pc & 0xff, b = (pc >> 8) & 0xff, c = pc >> 16, n = T.n;
+  if (T.D == 2) {
+    buf[a + n * b] = v;
+    buf[b + n * a] = v;
+  } else {
+    const int n2 = n * n;
+    buf[a + n * b + n2 * c] = v;
+    buf[a + n * c + n2 * b] = v;
+    buf[b + n * a + n2 * c] = v;
+    buf[b + n * c + n2 * a] = v;
+    buf[c + n * a + n2 * b] = v;
+    buf[c + n * b + n2 * a] = v;
   }
 }
 
 // ---------------------------------------------------------------------------
-// bubble: one workgroup per (nu, w) pair (the octant transform on all D axes, then
-// the wedge points read from it)
+// bubble: per (nu, w) pair the product of the two G tables on the octant, the
+// octant transform on all D axes, then the wedge points read from it.  A
+// workgroup takes a contiguous range of pairs (w-major, nu fastest) and gathers
+// the next pair's table values while it transforms the current one, so that
+// the gather latency hides behind the passes.
 // ---------------------------------------------------------------------------
-template <int MAXPT>
+template <int NQS, class PS>
 __global__ __launch_bounds__(kNT) void bubble_kernel(OctTables T, int Nnu, int n_inu_tot, int n_iw, int shift,
-                                                     int fft_first, const double2 *__restrict__ Gfft,
+                                                     int fft_first, int per_wg, const double2 *__restrict__ Gfft,
                                                      const double2 *__restrict__ Grfft, double2 *__restrict__ chi0,
                                                      double scale) {
   extern __shared__ double2 smem[];
   double2 *buf = smem;
   double *cs = reinterpret_cast<double *>(smem + T.NO);
-  const int pair = blockIdx.x;
-  const int nui = pair % Nnu, wi = pair / Nnu;
-  const int wn = wi - n_iw;
-  const int nun = -n_inu_tot + nui - trunc_half(shift * wn);
-  const size_t fs = (size_t)T.Nk;
-  const double2 *a = Gfft + (size_t)(nun - fft_first) * fs;
-  const double2 *b = Grfft + (size_t)(nun + wn - fft_first) * fs;
+  const int npairs = Nnu * (2 * n_iw + 1);
+  const int p0 = xcd_logical_block(blockIdx.x, gridDim.x) * per_wg, p1 = min(npairs, p0 + per_wg);
+  if (p0 >= p1) return;
   for (int m = threadIdx.x; m < T.Ns; m += kNT) cs[m] = T.cosT[m];
-  int w2o[kMaxQ];
-  load_w2o(T, w2o);  // used after the passes: their latency hides the loads
-  OctPts<MAXPT> P;
-  P.load(T);
-  // the products in batches of up to 10 points, every load of a batch in flight together
-  constexpr int B = MAXPT < 10 ? MAXPT : 10;
+  PS ps;
+  ps.init(T);
+  WedgeSlots<NQS> W;
+  W.load(T);
+  int f[NQS];
+  double2 pw[NQS];
 #pragma unroll
-  for (int p0 = 0; p0 < MAXPT; p0 += B) {
-    double2 va[B], vb[B];
-#pragma unroll
-    for (int k = 0; k < B; ++k) {
-      if (p0 + k < MAXPT) {
-        va[k] = a[P.f[p0 + k]];
-        vb[k] = b[P.f[p0 + k]];
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < B; ++k) {
-      const int o = threadIdx.x + (p0 + k) * kNT;
-      if (p0 + k < MAXPT && o < T.NO) buf[o] = cmul(cmul(va[k], vb[k]), P.pw[p0 + k]);
-    }
-  }
-  __syncthreads();
-  for (int ax = 0; ax < T.D; ++ax) oct_pass(buf, cs, ax, T);
-  double2 *out = chi0 + (size_t)T.Nq * (nui + (size_t)Nnu * wi);
-#pragma unroll
-  for (int k = 0; k < kMaxQ; ++k) {
+  for (int k = 0; k < NQS; ++k) {
     const int q = threadIdx.x + k * kNT;
-    if (q < T.Nq) out[q] = cscale(buf[w2o[k]], scale);
+    f[k] = W.valid(k) ? T.wedgefull[q] : 0;
+    pw[k] = W.valid(k) ? T.wedgepw[q] : cz(0.0, 0.0);
+  }
+  const size_t fs = (size_t)T.Nk;
+  double2 va[NQS], vb[NQS];
+  auto gather = [&](int P) {
+    const int nui = P % Nnu, wn = P / Nnu - n_iw;
+    const int nun = -n_inu_tot + nui - trunc_half(shift * wn);
+    const double2 *a = Gfft + (size_t)(nun - fft_first) * fs;
+    const double2 *b = Grfft + (size_t)(nun + wn - fft_first) * fs;
+#pragma unroll
+    for (int k = 0; k < NQS; ++k) {
+      va[k] = a[f[k]];
+      vb[k] = b[f[k]];
+    }
+  };
+  gather(p0);
+  for (int P = p0; P < p1; ++P) {
+#pragma unroll
+    for (int k = 0; k < NQS; ++k)
+      if (W.valid(k)) orbit_store(buf, T, W.pc[k], cmul(cmul(va[k], vb[k]), pw[k]));
+    if (P + 1 < p1) gather(P + 1);
+    __syncthreads();
+    for (int ax = 0; ax < T.D; ++ax) ps.run(buf, cs, ax, T);
+    double2 *out = chi0 + (size_t)T.Nq * P;  // chi0(q, nu, w), P = nu + Nnu w
+#pragma unroll
+    for (int k = 0; k < NQS; ++k) {
+      const int q = threadIdx.x + k * kNT;
+      if (W.valid(k)) out[q] = cscale(buf[W.oct(k, T)], scale);
+    }
+    __syncthreads();  // the reads of buf are done before the next pair's stores
   }
 }
 
@@ -198,7 +417,13 @@ __device__ __forceinline__ double2 eom_value(int kind, double U, double2 gm, dou
   }
 }
 
-template <int MAXPT>
+// K(q, nu, w) is formed on this thread's wedge points from gamma_m, gamma_d,
+// chi_m^lam_m, chi_d^lam_d and lambda0 (the lambda map applied in registers;
+// chi is never written), times the fold weight, and stored on each point's
+// orbit; after the passes the product with G_rfft(nu + w) is accumulated on the
+// wedge points, and the phase x weight of the back transform applied once at
+// the end.
+template <int NQS, class PS>
 __global__ __launch_bounds__(kNT) void eom_accum_kernel(OctTables T, int Nw, int Nnu, int n_inu, int n_iw, int shift,
                                                         int fft_first, int numax, int wchunk, int kind, double U,
                                                         double lam_m, double lam_d, const double *__restrict__ chim,
@@ -210,25 +435,25 @@ __global__ __launch_bounds__(kNT) void eom_accum_kernel(OctTables T, int Nw, int
                                                         double2 *__restrict__ partial) {
   extern __shared__ double2 smem[];
   double2 *buf = smem;
-  double2 *Kw = smem + T.NO;  // K on the wedge points
-  double *cs = reinterpret_cast<double *>(Kw + T.Nq);
-  const int nuii = blockIdx.x % numax;
-  const int chunk = blockIdx.x / numax;
+  double *cs = reinterpret_cast<double *>(smem + T.NO);
+  // neighbouring nu (which gather overlapping G planes) on one XCD
+  const int L = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int nuii = L % numax;
+  const int chunk = L / numax;
   for (int m = threadIdx.x; m < T.Ns; m += kNT) cs[m] = T.cosT[m];
-  double2 S[MAXPT];
+  PS ps;
+  ps.init(T);
+  WedgeSlots<NQS> W;
+  W.load(T);
+  int f[NQS];
+  double wq[NQS];
+  double2 S[NQS];
 #pragma unroll
-  for (int p = 0; p < MAXPT; ++p) S[p] = cz(0.0, 0.0);
-  // this thread's octant points: full-grid index, phase x weight, wedge index and
-  // weight, loaded once for the whole omega chunk
-  OctPts<MAXPT> P;
-  P.load(T);
-  int o2w[MAXPT];
-  double ow[MAXPT];
-#pragma unroll
-  for (int p = 0; p < MAXPT; ++p) {
-    const int o = threadIdx.x + p * kNT;
-    o2w[p] = o < T.NO ? T.oct2wedge[o] : 0;
-    ow[p] = o < T.NO ? T.octw[o] : 0.0;
+  for (int k = 0; k < NQS; ++k) {
+    const int q = threadIdx.x + k * kNT;
+    f[k] = W.valid(k) ? T.wedgefull[q] : 0;
+    wq[k] = W.valid(k) ? T.wedgew[q] : 0.0;
+    S[k] = cz(0.0, 0.0);
   }
   const int w0 = chunk * wchunk, w1 = min(Nw, w0 + wchunk);
   const size_t fs = (size_t)T.Nk;
@@ -237,43 +462,41 @@ __global__ __launch_bounds__(kNT) void eom_accum_kernel(OctTables T, int Nw, int
     // nu-window of gamma at this w: nu0 index (1-based) = n_inu + shift*trunc(wn/2) + 1  (helpers.jl:58)
     const int nui = n_inu + shift * trunc_half(wn) + nuii;  // 0-based index of nu_n = nuii
     if (nui > 2 * n_inu - 1) continue;                     // nu list ends at 2 n_inu (singleCore.jl:59-60)
-    // K on the wedge points, read once and coalesced (q is the fastest index of
-    // gamma, lambda0 and chi), then expanded to the weighted octant from LDS
-    for (int q = threadIdx.x; q < T.Nq; q += kNT) {
-      double cm = chim[q + (size_t)T.Nq * wi], cd = chid[q + (size_t)T.Nq * wi];
-      if (lam_m != 0.0) cm = cm / (lam_m * cm + 1.0);
-      if (lam_d != 0.0) cd = cd / (lam_d * cd + 1.0);
-      const size_t idx = q + (size_t)T.Nq * (nui + (size_t)Nnu * wi);
-      Kw[q] = eom_value(kind, U, gm[idx], gd[idx], cm, cd, l0[idx]);
-    }
-    __syncthreads();  // Kw written; the previous iteration is done with buf
+    // G(nu + w) at this thread's points, loaded first: the K loads and the passes hide the latency
+    const double2 *g = Grfft + (size_t)(nuii + wn - fft_first) * fs;
+    double2 gv[NQS];
 #pragma unroll
-    for (int p = 0; p < MAXPT; ++p) {
-      const int o = threadIdx.x + p * kNT;
-      if (o < T.NO) buf[o] = cscale(Kw[o2w[p]], ow[p]);
+    for (int k = 0; k < NQS; ++k) gv[k] = g[f[k]];
+    __syncthreads();  // the previous omega's reads of buf are done
+    // K on the wedge (q is the fastest index of gamma, lambda0 and chi: coalesced)
+#pragma unroll
+    for (int k = 0; k < NQS; ++k) {
+      const int q = threadIdx.x + k * kNT;
+      if (W.valid(k)) {
+        double cm = chim[q + (size_t)T.Nq * wi], cd = chid[q + (size_t)T.Nq * wi];
+        if (lam_m != 0.0) cm = cm / (lam_m * cm + 1.0);
+        if (lam_d != 0.0) cd = cd / (lam_d * cd + 1.0);
+        const size_t idx = q + (size_t)T.Nq * (nui + (size_t)Nnu * wi);
+        orbit_store(buf, T, W.pc[k], cscale(eom_value(kind, U, gm[idx], gd[idx], cm, cd, l0[idx]), wq[k]));
+      }
     }
     __syncthreads();
-    // G(nu + w) at this thread's points, loaded before the passes, which hide the latency
-    const double2 *g = Grfft + (size_t)(nuii + wn - fft_first) * fs;
-    double2 gv[MAXPT];
+    for (int ax = 0; ax < T.D; ++ax) ps.run(buf, cs, ax, T);
 #pragma unroll
-    for (int p = 0; p < MAXPT; ++p) gv[p] = g[P.f[p]];
-    for (int ax = 0; ax < T.D; ++ax) oct_pass(buf, cs, ax, T);
-#pragma unroll
-    for (int p = 0; p < MAXPT; ++p) {
-      const int o = threadIdx.x + p * kNT;
-      if (o < T.NO) S[p] = cadd(S[p], cmul(buf[o], cmul(gv[p], P.pw[p])));
-    }
+    for (int k = 0; k < NQS; ++k)
+      if (W.valid(k)) S[k] = cadd(S[k], cmul(buf[W.oct(k, T)], gv[k]));
   }
-  double2 *dst = partial + (size_t)T.NO * (nuii + (size_t)numax * chunk);
+  double2 *dst = partial + (size_t)T.Nq * (nuii + (size_t)numax * chunk);
 #pragma unroll
-  for (int p = 0; p < MAXPT; ++p) {
-    const int o = threadIdx.x + p * kNT;
-    if (o < T.NO) dst[o] = S[p];
+  for (int k = 0; k < NQS; ++k) {
+    const int q = threadIdx.x + k * kNT;
+    if (W.valid(k)) dst[q] = cmul(S[k], T.wedgepw[q]);
   }
 }
 
-template <int MAXPT>
+// Sums the chunk partials of one nu on the wedge, stores them on the orbits
+// and transforms back; adds the 1/(i nu) tail and the Hartree term.
+template <int NQS, class PS>
 __global__ __launch_bounds__(kNT) void eom_final_kernel(OctTables T, int numax, int nchunks,
                                                         const double2 *__restrict__ partial, double scale, double U,
                                                         double beta, double hartree, int add_tail,
@@ -284,34 +507,39 @@ __global__ __launch_bounds__(kNT) void eom_final_kernel(OctTables T, int numax, 
   double *cs = reinterpret_cast<double *>(smem + T.NO);
   const int nuii = blockIdx.x;
   for (int m = threadIdx.x; m < T.Ns; m += kNT) cs[m] = T.cosT[m];
-  int w2o[kMaxQ];
-  load_w2o(T, w2o);
+  PS ps;
+  ps.init(T);
+  WedgeSlots<NQS> W;
+  W.load(T);
   // the chunk partials of this nu, four chunks in flight per step
-  const size_t cstride = (size_t)T.NO * numax;
-  const double2 *pp = partial + (size_t)T.NO * nuii;
-  for (int o = threadIdx.x; o < T.NO; o += kNT) {
+  const size_t cstride = (size_t)T.Nq * numax;
+  const double2 *pp = partial + (size_t)T.Nq * nuii;
+#pragma unroll
+  for (int k = 0; k < NQS; ++k) {
+    const int q = threadIdx.x + k * kNT;
+    if (!W.valid(k)) continue;
     double2 s = cz(0.0, 0.0);
     int c = 0;
     for (; c + 4 <= nchunks; c += 4) {
-      const double2 a0 = pp[cstride * c + o], a1 = pp[cstride * (c + 1) + o];
-      const double2 a2 = pp[cstride * (c + 2) + o], a3 = pp[cstride * (c + 3) + o];
+      const double2 a0 = pp[cstride * c + q], a1 = pp[cstride * (c + 1) + q];
+      const double2 a2 = pp[cstride * (c + 2) + q], a3 = pp[cstride * (c + 3) + q];
       s = cadd(s, cadd(cadd(a0, a1), cadd(a2, a3)));
     }
-    for (; c < nchunks; ++c) s = cadd(s, pp[cstride * c + o]);
-    buf[o] = s;
+    for (; c < nchunks; ++c) s = cadd(s, pp[cstride * c + q]);
+    orbit_store(buf, T, W.pc[k], s);
   }
   __syncthreads();
-  for (int ax = 0; ax < T.D; ++ax) oct_pass(buf, cs, ax, T);
+  for (int ax = 0; ax < T.D; ++ax) ps.run(buf, cs, ax, T);
   // tail/(i nu_n) = -i * tail * beta / ((2n+1) pi)   (singleCore.jl:78-79)
   double tail = 0.0;
   if (add_tail) tail = -U * (tail_sum[0] - chiloc_m_sum);
   const double tail_im = -tail * beta / ((2.0 * nuii + 1.0) * M_PI);
   double2 *out = sigma + (size_t)T.Nq * nuii;
 #pragma unroll
-  for (int k = 0; k < kMaxQ; ++k) {
+  for (int k = 0; k < NQS; ++k) {
     const int q = threadIdx.x + k * kNT;
-    if (q < T.Nq) {
-      const double2 v = cscale(buf[w2o[k]], scale);
+    if (W.valid(k)) {
+      const double2 v = cscale(buf[W.oct(k, T)], scale);
       out[q] = cz(v.x + hartree, v.y + tail_im);
     }
   }
@@ -422,7 +650,7 @@ __global__ __launch_bounds__(kSumNT) void kintegrate_kernel(const double *__rest
 // (the derivatives of eom, src/ladderDGATools.jl:260, through chi/(lam chi + 1)).
 // Per workgroup: one nu, an omega chunk; partial sums of sum_q kMult K' T.
 // ---------------------------------------------------------------------------
-template <int MAXPT>
+template <int NQS, class PS>
 __global__ __launch_bounds__(kNT) void eom_adjoint_kernel(OctTables T, int Nw, int Nnu, int n_inu, int n_iw, int shift,
                                                           int fft_first, int numax, int wchunk, double U, double lam_m,
                                                           double lam_d, const double *__restrict__ chim,
@@ -436,20 +664,31 @@ __global__ __launch_bounds__(kNT) void eom_adjoint_kernel(OctTables T, int Nw, i
   double2 *buf = smem;
   double *cs = reinterpret_cast<double *>(smem + T.NO);
   __shared__ double2 red[2][kNT / 64];
-  const int nuii = blockIdx.x % numax;
-  const int chunk = blockIdx.x / numax;
+  // neighbouring nu (which gather overlapping G planes) on one XCD
+  const int L = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int nuii = L % numax;
+  const int chunk = L / numax;
   for (int m = threadIdx.x; m < T.Ns; m += kNT) cs[m] = T.cosT[m];
-  // forward transform of W(., nu) (weights w(q) of the octant fold)
-  for (int o = threadIdx.x; o < T.NO; o += kNT)
-    buf[o] = cscale(W[T.oct2wedge[o] + (size_t)T.Nq * nuii], T.octw[o]);
-  __syncthreads();
-  for (int ax = 0; ax < T.D; ++ax) oct_pass(buf, cs, ax, T);
-  double2 What[MAXPT];
+  PS ps;
+  ps.init(T);
+  WedgeSlots<NQS> P;
+  P.load(T);
+  int f[NQS];
 #pragma unroll
-  for (int p = 0; p < MAXPT; ++p) {
-    const int o = threadIdx.x + p * kNT;
-    What[p] = (o < T.NO) ? buf[o] : cz(0.0, 0.0);
+  for (int k = 0; k < NQS; ++k) f[k] = P.valid(k) ? T.wedgefull[threadIdx.x + k * kNT] : 0;
+  // forward transform of W(., nu) (weights w(q) of the octant fold)
+#pragma unroll
+  for (int k = 0; k < NQS; ++k) {
+    const int q = threadIdx.x + k * kNT;
+    if (P.valid(k)) orbit_store(buf, T, P.pc[k], cscale(W[q + (size_t)T.Nq * nuii], T.wedgew[q]));
   }
+  __syncthreads();
+  for (int ax = 0; ax < T.D; ++ax) ps.run(buf, cs, ax, T);
+  // W^ times the phase x weight of the second transform, on this thread's wedge points
+  double2 What[NQS];
+#pragma unroll
+  for (int k = 0; k < NQS; ++k)
+    What[k] = P.valid(k) ? cmul(buf[P.oct(k, T)], T.wedgepw[threadIdx.x + k * kNT]) : cz(0.0, 0.0);
   double2 am = cz(0.0, 0.0), ad = cz(0.0, 0.0);
   const int w0 = chunk * wchunk, w1 = min(Nw, w0 + wchunk);
   const size_t fs = (size_t)T.Nk;
@@ -458,16 +697,20 @@ __global__ __launch_bounds__(kNT) void eom_adjoint_kernel(OctTables T, int Nw, i
     const int nui = n_inu + shift * trunc_half(wn) + nuii;
     if (nui > 2 * n_inu - 1) continue;
     const double2 *g = Grfft + (size_t)(nuii + wn - fft_first) * fs;
+    double2 gv[NQS];
+#pragma unroll
+    for (int k = 0; k < NQS; ++k) gv[k] = g[f[k]];
     __syncthreads();  // previous users of buf are done
 #pragma unroll
-    for (int p = 0; p < MAXPT; ++p) {
-      const int o = threadIdx.x + p * kNT;
-      if (o < T.NO) buf[o] = cscale(cmul(What[p], cmul(g[T.octfull[o]], T.octphase[o])), T.octw[o]);
-    }
+    for (int k = 0; k < NQS; ++k)
+      if (P.valid(k)) orbit_store(buf, T, P.pc[k], cmul(What[k], gv[k]));
     __syncthreads();
-    for (int ax = 0; ax < T.D; ++ax) oct_pass(buf, cs, ax, T);
-    for (int q = threadIdx.x; q < T.Nq; q += kNT) {
-      const double2 t = cscale(buf[T.wedge2oct[q]], scale);
+    for (int ax = 0; ax < T.D; ++ax) ps.run(buf, cs, ax, T);
+#pragma unroll
+    for (int k = 0; k < NQS; ++k) {
+      const int q = threadIdx.x + k * kNT;
+      if (!P.valid(k)) continue;
+      const double2 t = cscale(buf[P.oct(k, T)], scale);
       double cm = chim[q + (size_t)T.Nq * wi], cd = chid[q + (size_t)T.Nq * wi];
       if (lam_m != 0.0) cm = cm / (lam_m * cm + 1.0);
       if (lam_d != 0.0) cd = cd / (lam_d * cd + 1.0);
@@ -536,7 +779,7 @@ __global__ void sum_pairs_kernel(int n, const double2 *__restrict__ partial, dou
 // even G), so G_rfft(R) = exp(2 pi i sum_d R_d / Ns) G_fft(R).  One workgroup
 // per nu: the octant in LDS, D passes, then both full grids written.
 // ---------------------------------------------------------------------------
-template <int MAXPT>
+template <class PS>
 __global__ __launch_bounds__(kNT) void gtables_kernel(OctTables T, const double2 *__restrict__ Gw,
                                                       double2 *__restrict__ Gfft, double2 *__restrict__ Grfft) {
   extern __shared__ double2 smem[];
@@ -545,9 +788,11 @@ __global__ __launch_bounds__(kNT) void gtables_kernel(OctTables T, const double2
   const int f = blockIdx.x;  // position in fft_range
   const double2 *g = Gw + (size_t)T.Nq * f;
   for (int m = threadIdx.x; m < T.Ns; m += kNT) cs[m] = T.cosT[m];
-  for (int o = threadIdx.x; o < T.NO; o += kNT) buf[o] = cscale(g[T.oct2wedge[o]], T.octw[o]);
+  PS ps;
+  ps.init(T);
+  for (int q = threadIdx.x; q < T.Nq; q += kNT) orbit_store(buf, T, T.wedgecoord[q], cscale(g[q], T.wedgew[q]));
   __syncthreads();
-  for (int ax = 0; ax < T.D; ++ax) oct_pass(buf, cs, ax, T);
+  for (int ax = 0; ax < T.D; ++ax) ps.run(buf, cs, ax, T);
   // the full grid, first axis fastest (column-major like the reference's arrays)
   double2 *a = Gfft + (size_t)T.Nk * f, *b = Grfft + (size_t)T.Nk * f;
   const int Nk = (int)T.Nk;
@@ -569,33 +814,56 @@ __global__ __launch_bounds__(kNT) void gtables_kernel(OctTables T, const double2
 
 }  // namespace
 
-// number of register slots per thread needed for the octant (NO / 256 rounded up)
-static int oct_maxpt(int NO) { return (NO + kNT - 1) / kNT; }
+// Launch an octant kernel (kNT threads); 3D octants need more than the
+// default 64 KiB of LDS.
+template <typename... KA, typename... A>
+static int oct_launch(ldga_ctx *ctx, void (*kern)(KA...), int grid, size_t smem, hipStream_t st, A... args) {
+  if (smem > 65536)
+    LDGA_HIP_TRY(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+  kern<<<grid, kNT, smem, st>>>(args...);
+  LDGA_HIP_TRY(ctx, hipGetLastError());
+  return LDGA_OK;
+}
 
-// Launch KERNEL<MAXPT> with the smallest register slot count that covers the
-// octant; large octants (3D) need more than the default 64 KiB of LDS.
-#define OCT_DISPATCH(NO, SMEM, KERNEL, GRID, STREAM, ...)                                          \
-  do {                                                                                             \
-    const int mp_ = oct_maxpt(NO);                                                                 \
-    auto go_ = [&](auto kern) -> int {                                                             \
-      if ((SMEM) > 65536)                                                                          \
-        LDGA_HIP_TRY(ctx, hipFuncSetAttribute((const void *)kern,                                  \
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)(SMEM))); \
-      kern<<<(GRID), kNT, (SMEM), (STREAM)>>>(__VA_ARGS__);                                        \
-      LDGA_HIP_TRY(ctx, hipGetLastError());                                                        \
-      return LDGA_OK;                                                                              \
-    };                                                                                             \
-    int rc_;                                                                                       \
-    if (mp_ <= 5)                                                                                  \
-      rc_ = go_(KERNEL<5>);                                                                        \
-    else if (mp_ <= 9)                                                                             \
-      rc_ = go_(KERNEL<9>);                                                                        \
-    else if (mp_ <= 20)                                                                            \
-      rc_ = go_(KERNEL<20>);                                                                       \
-    else                                                                                           \
-      rc_ = ldga_fail(ctx, LDGA_ERR_SHAPE, "octant of %d points is too large", NO);                \
-    if (rc_) return rc_;                                                                           \
-  } while (0)
+// workgroups of kern (kNT threads, smem bytes of LDS) resident on the whole
+// device at once; queried once per (kernel, smem) and cached in the context
+static int resident_wgs(ldga_ctx *ctx, const void *kern, size_t smem) {
+  const auto key = std::make_pair(kern, smem);
+  const auto it = ctx->resident.find(key);
+  if (it != ctx->resident.end()) return it->second;
+  int ncu = 0, per_cu = 0;
+  if (smem > 65536) (void)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) ncu = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kNT, smem) != hipSuccess) per_cu = 1;
+  const int r = std::max(1, ncu) * std::max(1, per_cu);
+  ctx->resident[key] = r;
+  return r;
+}
+
+// calls f(std::integral_constant<int, NQS>) for the wedge slot count of the k grid
+template <typename F>
+static int with_nqs(ldga_ctx *ctx, F &&f) {
+  switch ((ctx->oct.Nq + kNT - 1) / kNT) {
+    case 1:
+      return f(std::integral_constant<int, 1>{});
+    case 2:
+      return f(std::integral_constant<int, 2>{});
+    case 3:
+      return f(std::integral_constant<int, 3>{});
+    case 4:
+      return f(std::integral_constant<int, 4>{});
+    default:
+      return ldga_fail(ctx, LDGA_ERR_SHAPE, "wedge of %d points is too large", ctx->oct.Nq);
+  }
+}
+
+// calls f(std::integral_constant<int, NQS>, pass) with the pass kind of the k
+// grid: the matrix-core pass where it applies, the vector pass otherwise
+template <typename F>
+static int with_cfg(ldga_ctx *ctx, F &&f) {
+  const bool mf = mfma_pass_fits(ctx->oct.Ns);
+  return with_nqs(ctx, [&](auto c) { return mf ? f(c, MfmaPass{}) : f(c, ValuPass{}); });
+}
 
 int launch_bubble(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double2 *Gfft, const double2 *Grfft,
                   double2 *chi0, hipStream_t st) {
@@ -614,17 +882,22 @@ int launch_bubble(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double2
                      fr->fft_first + fr->n_fft - 1, lo, hi);
   const size_t smem = (size_t)T.NO * 16 + (size_t)T.Ns * 8;
   const double scale = -beta / (T.Nk * T.Nk);
-  OCT_DISPATCH(T.NO, smem, bubble_kernel, Nnu * Nw, st, T, Nnu, ntot, fr->n_iw, fr->shift, fr->fft_first, Gfft,
-               Grfft, chi0, scale);
-  return LDGA_OK;
+  // one round of resident workgroups, each over a contiguous range of pairs
+  return with_cfg(ctx, [&](auto c, auto ps) {
+    auto kern = bubble_kernel<decltype(c)::value, decltype(ps)>;
+    const int npairs = Nnu * Nw, res = resident_wgs(ctx, (const void *)kern, smem);
+    const int per_wg = (npairs + res - 1) / res;
+    return oct_launch(ctx, kern, (npairs + per_wg - 1) / per_wg, smem, st, T, Nnu, ntot, fr->n_iw, fr->shift,
+                      fr->fft_first, per_wg, Gfft, Grfft, chi0, scale);
+  });
 }
 
 int launch_gtables(ldga_ctx *ctx, const ldga_freq *fr, const double2 *Gw, double2 *Gfft, double2 *Grfft,
                    hipStream_t st) {
   const OctTables &T = ctx->oct;
   const size_t smem = (size_t)T.NO * 16 + (size_t)T.Ns * 8;
-  OCT_DISPATCH(T.NO, smem, gtables_kernel, fr->n_fft, st, T, Gw, Gfft, Grfft);
-  return LDGA_OK;
+  if (mfma_pass_fits(T.Ns)) return oct_launch(ctx, gtables_kernel<MfmaPass>, fr->n_fft, smem, st, T, Gw, Gfft, Grfft);
+  return oct_launch(ctx, gtables_kernel<ValuPass>, fr->n_fft, smem, st, T, Gw, Gfft, Grfft);
 }
 
 int launch_lambda0(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double2 *F, const double2 *chi0,
@@ -717,19 +990,13 @@ int launch_sum_w(ldga_ctx *ctx, const double *chi, int Nq, int Nw, int w_lo, int
 // round; each chunk covers wchunk consecutive omegas.
 int eom_chunks(ldga_ctx *ctx, int Nw, int numax, int *wchunk) {
   const OctTables &T = ctx->oct;
-  const int mp = oct_maxpt(T.NO);
-  const void *kern = mp <= 5 ? (const void *)eom_accum_kernel<5> : mp <= 9 ? (const void *)eom_accum_kernel<9>
-                                                                             : (const void *)eom_accum_kernel<20>;
-  const size_t smem = (size_t)T.NO * 16 + (size_t)T.Ns * 8 + (size_t)T.Nq * 16;
-  if (ctx->eom_resident_key != (size_t)T.NO * 4096 + T.Nq) {
-    int ncu = 0, per_cu = 0;
-    if (smem > 65536) hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device);
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kNT, smem);
-    ctx->eom_resident = std::max(1, ncu) * std::max(1, per_cu);
-    ctx->eom_resident_key = (size_t)T.NO * 4096 + T.Nq;
-  }
-  int nchunks = std::max(1, std::min(Nw, ctx->eom_resident / std::max(numax, 1)));
+  const size_t smem = (size_t)T.NO * 16 + (size_t)T.Ns * 8;
+  int resident = 1;
+  with_cfg(ctx, [&](auto c, auto ps) {
+    resident = resident_wgs(ctx, (const void *)eom_accum_kernel<decltype(c)::value, decltype(ps)>, smem);
+    return LDGA_OK;
+  });
+  int nchunks = std::max(1, std::min(Nw, resident / std::max(numax, 1)));
   *wchunk = (Nw + nchunks - 1) / nchunks;
   return (Nw + *wchunk - 1) / *wchunk;
 }
@@ -750,7 +1017,7 @@ int launch_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int n
     return ldga_fail(ctx, LDGA_ERR_SHAPE, "nu window does not contain nu_n = 0 at every w");
   int wchunk = 1;
   const int nchunks = eom_chunks(ctx, Nw, numax, &wchunk);
-  double2 *partial = (double2 *)ldga_scratch(ctx, "eom_partial", (size_t)T.NO * numax * nchunks * 16, st);
+  double2 *partial = (double2 *)ldga_scratch(ctx, "eom_partial", (size_t)T.Nq * numax * nchunks * 16, st);
   double *tsum = (double *)ldga_scratch(ctx, "eom_tail", 3 * 8, st);
   if (!partial || !tsum) return LDGA_ERR_HIP;
   const bool add_tail = tc && eom_kind == LDGA_EOM_FULL;
@@ -759,15 +1026,18 @@ int launch_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int n
     if (rc) return rc;
   }
   const size_t smem = (size_t)T.NO * 16 + (size_t)T.Ns * 8;
-  const size_t smem_acc = smem + (size_t)T.Nq * 16;
-  OCT_DISPATCH(T.NO, smem_acc, eom_accum_kernel, numax * nchunks, st, T, Nw, Nnu, fr->n_inu, fr->n_iw, fr->shift,
-               fr->fft_first, numax, wchunk, eom_kind, mp->U, lam_m, lam_d, chi_m, chi_d, gamma_m, gamma_d, lambda0,
-               Grfft, partial);
+  int rc = with_cfg(ctx, [&](auto c, auto ps) {
+    return oct_launch(ctx, eom_accum_kernel<decltype(c)::value, decltype(ps)>, numax * nchunks, smem, st, T, Nw, Nnu, fr->n_inu,
+                      fr->n_iw, fr->shift, fr->fft_first, numax, wchunk, eom_kind, mp->U, lam_m, lam_d, chi_m, chi_d,
+                      gamma_m, gamma_d, lambda0, Grfft, partial);
+  });
+  if (rc) return rc;
   const double hartree = (eom_kind == LDGA_EOM_FULL || eom_kind == LDGA_EOM_REST) ? mp->n * mp->U / 2.0 : 0.0;
   const double scale = 1.0 / (mp->beta * T.Nk * T.Nk);
-  OCT_DISPATCH(T.NO, smem, eom_final_kernel, numax, st, T, numax, nchunks, partial, scale, mp->U, mp->beta, hartree,
-               add_tail ? 1 : 0, tsum, chiloc_m_sum, sigma);
-  return LDGA_OK;
+  return with_cfg(ctx, [&](auto c, auto ps) {
+    return oct_launch(ctx, eom_final_kernel<decltype(c)::value, decltype(ps)>, numax, smem, st, T, numax, nchunks, partial, scale,
+                      mp->U, mp->beta, hartree, add_tail ? 1 : 0, tsum, chiloc_m_sum, sigma);
+  });
 }
 
 // Adjoint contraction sum_{nu,w,q} kMult K'_r T / beta for r = m, d into out_dev[0..1] (complex).
@@ -783,9 +1053,12 @@ int launch_eom_adjoint(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp,
   if (!partial) return LDGA_ERR_HIP;
   const size_t smem = (size_t)T.NO * 16 + (size_t)T.Ns * 8;
   const double scale = 1.0 / (T.Nk * T.Nk);
-  OCT_DISPATCH(T.NO, smem, eom_adjoint_kernel, numax * nchunks, st, T, Nw, Nnu, fr->n_inu, fr->n_iw, fr->shift,
-               fr->fft_first, numax, wchunk, mp->U, lam_m, lam_d, chi_m, chi_d, gamma_m, gamma_d, Grfft, W, scale,
-               partial);
+  const int rc = with_cfg(ctx, [&](auto c, auto ps) {
+    return oct_launch(ctx, eom_adjoint_kernel<decltype(c)::value, decltype(ps)>, numax * nchunks, smem, st, T, Nw, Nnu, fr->n_inu,
+                      fr->n_iw, fr->shift, fr->fft_first, numax, wchunk, mp->U, lam_m, lam_d, chi_m, chi_d, gamma_m,
+                      gamma_d, Grfft, W, scale, partial);
+  });
+  if (rc) return rc;
   sum_pairs_kernel<<<1, 256, 0, st>>>(numax * nchunks, partial, out_dev);
   LDGA_HIP_TRY(ctx, hipGetLastError());
   return LDGA_OK;

@@ -36,12 +36,14 @@ __host__ __device__ inline double2 cdiv(double2 a, double2 b) { return cmul(a, c
 // ---------------------------------------------------------------------------
 struct OctTables {
   int D, Ns, n, NO, Nq;
-  const int32_t *oct2wedge;  // NO: wedge index of the (sorted) octant point
-  const int32_t *wedge2oct;  // Nq: octant linear index of wedge point q
-  const int32_t *octfull;    // NO: linear index of the octant point in the full Ns^D grid
-  const double *octw;        // NO: multiplicity weight prod_d w(R_d), w = 1 at 0 and Ns/2, else 2
-  const double2 *octphase;   // NO: exp(-2 pi i sum_d R_d / Ns), undoes the reverse() of G_rfft
-  const double2 *octpw;      // NO: octphase * octw
+  // Per wedge point q (coordinates c0 >= c1 >= c2 in 0..Ns/2).  Every function
+  // of the path is symmetric under permutations of the axes, so the octant
+  // points of q's orbit (its coordinates permuted) hold q's value.
+  const int32_t *wedgecoord;  // Nq: c0 | c1 << 8 | c2 << 16
+  const int32_t *wedgefull;   // Nq: linear index of (c0, c1, c2) in the full Ns^D grid
+  const double *wedgew;       // Nq: fold weight prod_d w(c_d), w = 1 at 0 and Ns/2, else 2
+  const double2 *wedgephase;  // Nq: exp(-2 pi i sum_d c_d / Ns), undoes the reverse() of G_rfft
+  const double2 *wedgepw;     // Nq: wedgephase * wedgew
   const double *cosT;        // Ns: cos(2 pi m / Ns)
   const double *kmult;       // Nq: wedge multiplicities
   const double *eps;         // Nq: dispersion on the wedge
@@ -80,9 +82,8 @@ struct ldga_ctx {
   // device address of this device's copy of the allocation-free entry
   // point's static scratch array (set by ldga_create)
   void *noalloc_scratch = nullptr;
-  // resident eom_accum workgroups on the device, for the grid size it was found for
-  int eom_resident = 0;
-  size_t eom_resident_key = 0;
+  // workgroups of a kernel resident on the whole device at once, by (kernel, LDS bytes)
+  std::map<std::pair<const void *, size_t>, int> resident;
 };
 
 // looks up the static scratch array of the allocation-free entry point on the
