@@ -205,8 +205,8 @@ __device__ __forceinline__ double row_sum4(double v) {
   return __hiloint2double(h16[0], l16[0]) + __hiloint2double(h16[1], l16[1]);
 }
 
-// The folded pass on the FP64 matrix cores, for Ns = 32, 48, 64 (g = Ns/4 a
-// multiple of 4, at most 16).  A wave takes 8 lines (16 real columns: the
+// The folded pass on the FP64 matrix cores, for Ns = 16 NKB (NKB = 2, 3, 4:
+// Ns = 32, 48, 64; g = Ns/4 = 4 NKB).  A wave takes 8 lines (16 real columns: the
 // real and imaginary parts) at a time.  With u and v of the fold,
 //   y[2m]   = sum_{r<g} C1[m][r] u[r] + (-1)^m x[g]     C1[m][r] = cos(pi 2m r / h)
 //   y[2m+1] = sum_{r<g} C2[m][r] v[r]                   C2[m][r] = cos(pi (2m+1) r / h)
@@ -218,12 +218,14 @@ __device__ __forceinline__ double row_sum4(double v) {
 // element e of lane l is row l / 16 + 4 e, column l % 16.  A wave reads and
 // writes only its own lines, so a pass needs one barrier, at its end.
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+template <int NKB>
 struct MfmaPass {
-  double ae[4], ao[4];
+  static constexpr int Ns = 16 * NKB, h = Ns / 2, g = h / 2;
+  double ae[NKB], ao[NKB];
   __device__ __forceinline__ void init(const OctTables &T) {
-    const int lane = threadIdx.x & 63, i = lane & 15, k = lane >> 4, Ns = T.Ns, g = Ns / 4;
+    const int lane = threadIdx.x & 63, i = lane & 15, k = lane >> 4;
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb) {
+    for (int kb = 0; kb < NKB; ++kb) {
       const int r = 4 * kb + k;
       const bool v = i < g && r < g;
       ae[kb] = v ? T.cosT[(2 * i * r) % Ns] : 0.0;
@@ -231,7 +233,8 @@ struct MfmaPass {
     }
   }
   __device__ __forceinline__ void run(double2 *buf, const double *, int ax, const OctTables &T) const {
-    const int n = T.n, h = T.Ns / 2, g = h / 2, nkb = g / 4, stride = T.stride[ax], nlines = T.NO / n;
+    constexpr int n = h + 1;
+    const int stride = T.stride[ax], nlines = T.NO / n;
     const int nct = (nlines + 7) / 8;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j = lane & 15, k = lane >> 4, part = j & 1;
@@ -248,34 +251,35 @@ struct MfmaPass {
       else if (ax == T.D - 1)
         base = lc;
       else {  // 3D, axis 1: line = a + n c
-        const int c = (int)(((float)lc + 0.5f) / (float)n);
+        const int c = (int)(((float)lc + 0.5f) * (1.0f / n));
         base = lc + c * (n * n - n);
       }
       const int step = 8 * stride;  // doubles per 4 points
       const int plo = 2 * (base + k * stride) + part, phi = 2 * (base + (h - k) * stride) + part;
+      double x0[NKB], x1[NKB];
+#pragma unroll
+      for (int kb = 0; kb < NKB; ++kb) {
+        x0[kb] = bd[plo + kb * step];
+        x1[kb] = bd[phi - kb * step];
+      }
+      const double xg = bd[2 * (base + g * stride) + part];
       dbl4 de = {0.0, 0.0, 0.0, 0.0}, dd = {0.0, 0.0, 0.0, 0.0};
       double s = 0.0;
 #pragma unroll
-      for (int kb = 0; kb < 4; ++kb) {
-        if (kb < nkb) {
-          const double x0 = bd[plo + kb * step], x1 = bd[phi - kb * step];
-          const double u = x0 + x1, v = x0 - x1;
-          de = __builtin_amdgcn_mfma_f64_16x16x4f64(ae[kb], u, de, 0, 0, 0);
-          dd = __builtin_amdgcn_mfma_f64_16x16x4f64(ao[kb], v, dd, 0, 0, 0);
-          s += u;
-        }
+      for (int kb = 0; kb < NKB; ++kb) {
+        const double u = x0[kb] + x1[kb], v = x0[kb] - x1[kb];
+        de = __builtin_amdgcn_mfma_f64_16x16x4f64(ae[kb], u, de, 0, 0, 0);
+        dd = __builtin_amdgcn_mfma_f64_16x16x4f64(ao[kb], v, dd, 0, 0, 0);
+        s += u;
       }
-      const double xg = bd[2 * (base + g * stride) + part];
       const double xs = (k & 1) ? -xg : xg;  // (-1)^m x[g] for the rows m = k + 4e
       s = row_sum4((k & 1) ? -s : s);       // (-1)^r = (-1)^k for r = 4 kb + k
       if (lv) {
         const int w = 2 * (base + 2 * k * stride) + part, wstep = 16 * stride;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          if (4 * e < g) {
-            bd[w + e * wstep] = de[e] + xs;
-            bd[w + 2 * stride + e * wstep] = dd[e];
-          }
+        for (int e = 0; e < NKB; ++e) {  // rows m = k + 4e < g
+          bd[w + e * wstep] = de[e] + xs;
+          bd[w + 2 * stride + e * wstep] = dd[e];
         }
         if (k == 0) bd[2 * (base + h * stride) + part] = s + ((g & 1) ? -xg : xg);
       }
@@ -284,8 +288,6 @@ struct MfmaPass {
   }
 };
 
-// the matrix-core pass applies to the grid
-__host__ __device__ inline bool mfma_pass_fits(int Ns) { return Ns == 32 || Ns == 48 || Ns == 64; }
 
 __device__ __forceinline__ int trunc_half(int x) { return x / 2; }  // C division truncates toward 0 like Julia trunc
 
@@ -858,11 +860,20 @@ static int with_nqs(ldga_ctx *ctx, F &&f) {
 }
 
 // calls f(std::integral_constant<int, NQS>, pass) with the pass kind of the k
-// grid: the matrix-core pass where it applies, the vector pass otherwise
+// grid: the matrix-core pass for Ns = 32, 48, 64 (the wedge sizes of those
+// grids fix NQS), the vector pass otherwise
 template <typename F>
 static int with_cfg(ldga_ctx *ctx, F &&f) {
-  const bool mf = mfma_pass_fits(ctx->oct.Ns);
-  return with_nqs(ctx, [&](auto c) { return mf ? f(c, MfmaPass{}) : f(c, ValuPass{}); });
+  const int Ns = ctx->oct.Ns, D = ctx->oct.D, nqs = (ctx->oct.Nq + kNT - 1) / kNT;
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  using I4 = std::integral_constant<int, 4>;
+  if (D == 2 && Ns == 64 && nqs == 3) return f(I3{}, MfmaPass<4>{});
+  if (D == 2 && Ns == 48 && nqs == 2) return f(I2{}, MfmaPass<3>{});
+  if (D == 2 && Ns == 32 && nqs == 1) return f(I1{}, MfmaPass<2>{});
+  if (D == 3 && Ns == 32 && nqs == 4) return f(I4{}, MfmaPass<2>{});
+  return with_nqs(ctx, [&](auto c) { return f(c, ValuPass{}); });
 }
 
 int launch_bubble(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double2 *Gfft, const double2 *Grfft,
@@ -896,8 +907,9 @@ int launch_gtables(ldga_ctx *ctx, const ldga_freq *fr, const double2 *Gw, double
                    hipStream_t st) {
   const OctTables &T = ctx->oct;
   const size_t smem = (size_t)T.NO * 16 + (size_t)T.Ns * 8;
-  if (mfma_pass_fits(T.Ns)) return oct_launch(ctx, gtables_kernel<MfmaPass>, fr->n_fft, smem, st, T, Gw, Gfft, Grfft);
-  return oct_launch(ctx, gtables_kernel<ValuPass>, fr->n_fft, smem, st, T, Gw, Gfft, Grfft);
+  return with_cfg(ctx, [&](auto, auto ps) {
+    return oct_launch(ctx, gtables_kernel<decltype(ps)>, fr->n_fft, smem, st, T, Gw, Gfft, Grfft);
+  });
 }
 
 int launch_lambda0(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double2 *F, const double2 *chi0,
