@@ -193,6 +193,99 @@ struct ValuPass {
   }
 };
 
+// The folded pass with one real column (the real or imaginary part of a line)
+// per lane and the whole line in registers, for Ns = NS divisible by 4: with
+// the fold u, v of the line,
+//   y[2m]   = sum_{r<=g} cos(2 pi 2m r / Ns) u[r]      (pairs m, g - m: E +- O)
+//   y[2m+1] = sum_{r<g}  cos(2 pi (2m+1) r / Ns) v[r]  (pairs m, g - 1 - m)
+// with E, O the sums over even and odd r.  Every index is a compile-time
+// constant: the cosines are wave-uniform (the quarter table cos(2 pi j / Ns),
+// j <= Ns/4, held in scalar registers, signs folded into the multiply-adds;
+// the zero at j = Ns/4 skipped) and there is no recurrence.  A lane reads and
+// writes only its own column, so the pass has one barrier, at its end.
+template <int NS>
+struct RegPass {
+  static constexpr int h = NS / 2, n = h + 1, g = h / 2, nc = NS / 4;
+  double cv[nc];  // cos(2 pi j / NS), j = 0 .. NS/4 - 1
+  __device__ __forceinline__ void init(const OctTables &T) {
+#pragma unroll
+    for (int j = 0; j < nc; ++j) cv[j] = T.cosT[j];
+  }
+  static constexpr int fold_j(int j) {  // |index| into the quarter table, or nc for cos = 0
+    j %= NS;
+    j = j > NS / 2 ? NS - j : j;
+    return j > NS / 4 ? NS / 2 - j : j;
+  }
+  static constexpr bool neg_j(int j) {
+    j %= NS;
+    j = j > NS / 2 ? NS - j : j;
+    return j > NS / 4;
+  }
+  // acc + cos(2 pi j / NS) x
+  __device__ __forceinline__ double mac(int j, double x, double acc) const {
+    const int f = fold_j(j);
+    if (f == nc) return acc;  // cos(pi / 2) = 0
+    if (f == 0) return neg_j(j) ? acc - x : acc + x;
+    return fma(neg_j(j) ? -cv[f] : cv[f], x, acc);
+  }
+  __device__ __forceinline__ void run(double2 *buf, const double *, int ax, const OctTables &T) const {
+    const int stride = T.stride[ax], nlines = T.NO / n, ncol = 2 * nlines, st = 2 * stride;
+    double *bd = reinterpret_cast<double *>(buf);
+    // the columns of a partial round go to different waves (SIMDs) in each
+    // pass and workgroup
+    constexpr int nw = kNT / 64;
+    const int lt = (((threadIdx.x >> 6) + blockIdx.x + ax) & (nw - 1)) * 64 + (threadIdx.x & 63);
+    for (int col = lt; col < ncol; col += kNT) {
+      const int line = col >> 1;
+      int base;  // octant index of the line's point 0
+      if (ax == 0)
+        base = line * n;
+      else if (ax == T.D - 1)
+        base = line;
+      else {  // 3D, axis 1: line = a + n c
+        const int c = (int)(((float)line + 0.5f) * (1.0f / n));
+        base = line + c * (n * n - n);
+      }
+      double *p = bd + 2 * base + (col & 1);
+      double u[g + 1], v[g];
+#pragma unroll
+      for (int r = 0; r < g; ++r) {
+        const double a = p[r * st], b = p[(h - r) * st];
+        u[r] = a + b;
+        v[r] = a - b;
+      }
+      u[g] = p[g * st];
+#pragma unroll
+      for (int m = 0; 2 * m <= g; ++m) {
+        double E = u[0], O = 0.0;
+#pragma unroll
+        for (int r = 1; r <= g; ++r) {
+          if (r % 2 == 0)
+            E = mac(2 * m * r, u[r], E);
+          else
+            O = mac(2 * m * r, u[r], O);
+        }
+        p[2 * m * st] = E + O;
+        if (2 * m != g) p[(h - 2 * m) * st] = E - O;
+      }
+#pragma unroll
+      for (int m = 0; 2 * m < g; ++m) {
+        double E = v[0], O = 0.0;
+#pragma unroll
+        for (int r = 1; r < g; ++r) {
+          if (r % 2 == 0)
+            E = mac((2 * m + 1) * r, v[r], E);
+          else
+            O = mac((2 * m + 1) * r, v[r], O);
+        }
+        p[(2 * m + 1) * st] = E + O;
+        if (4 * m + 2 != h) p[(h - 2 * m - 1) * st] = E - O;
+      }
+    }
+    __syncthreads();
+  }
+};
+
 // sum of v over the four 16-lane rows of the wave, in every lane (permlane swaps)
 __device__ __forceinline__ double row_sum4(double v) {
   const unsigned lo = __double2loint(v), hi = __double2hiint(v);
@@ -859,9 +952,12 @@ static int with_nqs(ldga_ctx *ctx, F &&f) {
   }
 }
 
-// calls f(std::integral_constant<int, NQS>, pass) with the pass kind of the k
-// grid: the matrix-core pass for Ns = 32, 48, 64 (the wedge sizes of those
-// grids fix NQS), the vector pass otherwise
+// calls f(std::integral_constant<int, NQS>, pass) with the pass of the k grid
+// (the wedge size of each grid below fixes NQS): the matrix-core pass for the
+// 2D grids with 33 and 25 points a line (their 66 and 50 real columns fill a
+// wave's 16-column tiles better than one column per lane), the
+// register-resident pass for the other common Ns, the general vector pass
+// otherwise
 template <typename F>
 static int with_cfg(ldga_ctx *ctx, F &&f) {
   const int Ns = ctx->oct.Ns, D = ctx->oct.D, nqs = (ctx->oct.Nq + kNT - 1) / kNT;
@@ -869,10 +965,15 @@ static int with_cfg(ldga_ctx *ctx, F &&f) {
   using I2 = std::integral_constant<int, 2>;
   using I3 = std::integral_constant<int, 3>;
   using I4 = std::integral_constant<int, 4>;
+  // the wedge of each (D, Ns) below has the slot count named with it
   if (D == 2 && Ns == 64 && nqs == 3) return f(I3{}, MfmaPass<4>{});
   if (D == 2 && Ns == 48 && nqs == 2) return f(I2{}, MfmaPass<3>{});
-  if (D == 2 && Ns == 32 && nqs == 1) return f(I1{}, MfmaPass<2>{});
-  if (D == 3 && Ns == 32 && nqs == 4) return f(I4{}, MfmaPass<2>{});
+  if (D == 2 && Ns == 32 && nqs == 1) return f(I1{}, RegPass<32>{});
+  if (D == 2 && Ns == 24 && nqs == 1) return f(I1{}, RegPass<24>{});
+  if (D == 2 && Ns == 16 && nqs == 1) return f(I1{}, RegPass<16>{});
+  if (D == 3 && Ns == 32 && nqs == 4) return f(I4{}, RegPass<32>{});
+  if (D == 3 && Ns == 24 && nqs == 2) return f(I2{}, RegPass<24>{});
+  if (D == 3 && Ns == 16 && nqs == 1) return f(I1{}, RegPass<16>{});
   return with_nqs(ctx, [&](auto c) { return f(c, ValuPass{}); });
 }
 
