@@ -185,12 +185,22 @@ __device__ __forceinline__ void oct_pass(double2 *buf, const double *cs, int ax,
     oct_pass_paired(buf, cs, ax, T);
 }
 
+// octant index of the point with packed coordinates pc (c0 | c1 << 8 | c2 << 16)
+__device__ __forceinline__ int oct_of(int pc, const OctTables &T) {
+  return (pc & 0xff) + T.n * (((pc >> 8) & 0xff) + T.n * (pc >> 16));
+}
+
 // The passes on the vector ALU (any even Ns).
 struct ValuPass {
   __device__ __forceinline__ void init(const OctTables &) {}
   __device__ __forceinline__ void run(double2 *buf, const double *cs, int ax, const OctTables &T) const {
     oct_pass(buf, cs, ax, T);
   }
+  // all D passes, for outputs read at wedge points only (oct_out)
+  __device__ __forceinline__ void transform(double2 *buf, const double *cs, const OctTables &T) const {
+    for (int ax = 0; ax < T.D; ++ax) run(buf, cs, ax, T);
+  }
+  __device__ __forceinline__ int oct_out(int pc, const OctTables &T) const { return oct_of(pc, T); }
 };
 
 // The folded pass with one real column (the real or imaginary part of a line)
@@ -228,14 +238,53 @@ struct RegPass {
     if (f == 0) return neg_j(j) ? acc - x : acc + x;
     return fma(neg_j(j) ? -cv[f] : cv[f], x, acc);
   }
-  __device__ __forceinline__ void run(double2 *buf, const double *, int ax, const OctTables &T) const {
-    const int stride = T.stride[ax], nlines = T.NO / n, ncol = 2 * nlines, st = 2 * stride;
-    double *bd = reinterpret_cast<double *>(buf);
-    // the columns of a partial round go to different waves (SIMDs) in each
-    // pass and workgroup
+  // the transform of one real column: point r at p[r st]
+  __device__ __forceinline__ void column(double *p, int st) const {
+    double u[g + 1], v[g];
+#pragma unroll
+    for (int r = 0; r < g; ++r) {
+      const double a = p[r * st], b = p[(h - r) * st];
+      u[r] = a + b;
+      v[r] = a - b;
+    }
+    u[g] = p[g * st];
+#pragma unroll
+    for (int m = 0; 2 * m <= g; ++m) {
+      double E = u[0], O = 0.0;
+#pragma unroll
+      for (int r = 1; r <= g; ++r) {
+        if (r % 2 == 0)
+          E = mac(2 * m * r, u[r], E);
+        else
+          O = mac(2 * m * r, u[r], O);
+      }
+      p[2 * m * st] = E + O;
+      if (2 * m != g) p[(h - 2 * m) * st] = E - O;
+    }
+#pragma unroll
+    for (int m = 0; 2 * m < g; ++m) {
+      double E = v[0], O = 0.0;
+#pragma unroll
+      for (int r = 1; r < g; ++r) {
+        if (r % 2 == 0)
+          E = mac((2 * m + 1) * r, v[r], E);
+        else
+          O = mac((2 * m + 1) * r, v[r], O);
+      }
+      p[(2 * m + 1) * st] = E + O;
+      if (4 * m + 2 != h) p[(h - 2 * m - 1) * st] = E - O;
+    }
+  }
+  // this thread's index among the columns of a round: the columns of a partial
+  // round go to different waves (SIMDs) in each pass and workgroup
+  __device__ __forceinline__ int col0(int ax) const {
     constexpr int nw = kNT / 64;
-    const int lt = (((threadIdx.x >> 6) + blockIdx.x + ax) & (nw - 1)) * 64 + (threadIdx.x & 63);
-    for (int col = lt; col < ncol; col += kNT) {
+    return (((threadIdx.x >> 6) + blockIdx.x + ax) & (nw - 1)) * 64 + (threadIdx.x & 63);
+  }
+  __device__ __forceinline__ void run(double2 *buf, const double *, int ax, const OctTables &T) const {
+    const int stride = T.stride[ax], nlines = T.NO / n, ncol = 2 * nlines;
+    double *bd = reinterpret_cast<double *>(buf);
+    for (int col = col0(ax); col < ncol; col += kNT) {
       const int line = col >> 1;
       int base;  // octant index of the line's point 0
       if (ax == 0)
@@ -246,43 +295,46 @@ struct RegPass {
         const int c = (int)(((float)line + 0.5f) * (1.0f / n));
         base = line + c * (n * n - n);
       }
-      double *p = bd + 2 * base + (col & 1);
-      double u[g + 1], v[g];
+      column(bd + 2 * base + (col & 1), 2 * stride);
+    }
+    __syncthreads();
+  }
+  // All D passes for outputs read at wedge points (oct_out).  The output is
+  // symmetric under permutations of the axes, so the last pass transforms only
+  // the lines the wedge reads need: in 3D the lines (x, y) with x >= y (the
+  // wedge point (c0, c1, c2) is read at x = c0, y = c1, z = c2); in 2D the
+  // lines x < h, the points (h, c1) being read at (c1, h), and the corner
+  // (h, h) = sum_y (-1)^y T(h, y) of the first pass's output summed apart.
+  __device__ __forceinline__ void transform(double2 *buf, const double *cs, const OctTables &T) const {
+    for (int ax = 0; ax + 1 < T.D; ++ax) run(buf, cs, ax, T);
+    const int ax = T.D - 1;
+    double *bd = reinterpret_cast<double *>(buf);
+    if (T.D == 2) {
+      const int c = col0(ax);
+      if (c < 2 * h) {
+        column(bd + 2 * (c >> 1) + (c & 1), 2 * n);
+      } else if (c < 2 * h + 2) {  // the corner, one lane per part, in a wave with no column
+        const int part = c & 1;
+        double s = 0.0;
 #pragma unroll
-      for (int r = 0; r < g; ++r) {
-        const double a = p[r * st], b = p[(h - r) * st];
-        u[r] = a + b;
-        v[r] = a - b;
+        for (int y = 0; y <= h; ++y) s += (y & 1) ? -bd[2 * (h + n * y) + part] : bd[2 * (h + n * y) + part];
+        bd[2 * (h + n * h) + part] = s;
       }
-      u[g] = p[g * st];
-#pragma unroll
-      for (int m = 0; 2 * m <= g; ++m) {
-        double E = u[0], O = 0.0;
-#pragma unroll
-        for (int r = 1; r <= g; ++r) {
-          if (r % 2 == 0)
-            E = mac(2 * m * r, u[r], E);
-          else
-            O = mac(2 * m * r, u[r], O);
-        }
-        p[2 * m * st] = E + O;
-        if (2 * m != g) p[(h - 2 * m) * st] = E - O;
-      }
-#pragma unroll
-      for (int m = 0; 2 * m < g; ++m) {
-        double E = v[0], O = 0.0;
-#pragma unroll
-        for (int r = 1; r < g; ++r) {
-          if (r % 2 == 0)
-            E = mac((2 * m + 1) * r, v[r], E);
-          else
-            O = mac((2 * m + 1) * r, v[r], O);
-        }
-        p[(2 * m + 1) * st] = E + O;
-        if (4 * m + 2 != h) p[(h - 2 * m - 1) * st] = E - O;
+    } else {
+      constexpr int nl = n * (n + 1) / 2;
+      for (int col = col0(ax); col < 2 * nl; col += kNT) {
+        const int l = col >> 1;
+        const int x = (int)((sqrtf(8.0f * (float)l + 1.0f) - 1.0f) * 0.5f);
+        const int y = l - x * (x + 1) / 2;  // l = x (x + 1) / 2 + y, y <= x
+        column(bd + 2 * (x + n * y) + (col & 1), 2 * n * n);
       }
     }
     __syncthreads();
+  }
+  __device__ __forceinline__ int oct_out(int pc, const OctTables &T) const {
+    const int a = pc & 0xff, b = (pc >> 8) & 0xff;
+    if (T.D == 2) return a == h ? b + n * h : a + n * b;
+    return oct_of(pc, T);
   }
 };
 
@@ -379,6 +431,10 @@ struct MfmaPass {
     }
     __syncthreads();
   }
+  __device__ __forceinline__ void transform(double2 *buf, const double *cs, const OctTables &T) const {
+    for (int ax = 0; ax < T.D; ++ax) run(buf, cs, ax, T);
+  }
+  __device__ __forceinline__ int oct_out(int pc, const OctTables &T) const { return oct_of(pc, T); }
 };
 
 
@@ -475,12 +531,12 @@ __global__ __launch_bounds__(kNT) void bubble_kernel(OctTables T, int Nnu, int n
       if (W.valid(k)) orbit_store(buf, T, W.pc[k], cmul(cmul(va[k], vb[k]), pw[k]));
     if (P + 1 < p1) gather(P + 1);
     __syncthreads();
-    for (int ax = 0; ax < T.D; ++ax) ps.run(buf, cs, ax, T);
+    ps.transform(buf, cs, T);
     double2 *out = chi0 + (size_t)T.Nq * P;  // chi0(q, nu, w), P = nu + Nnu w
 #pragma unroll
     for (int k = 0; k < NQS; ++k) {
       const int q = threadIdx.x + k * kNT;
-      if (W.valid(k)) out[q] = cscale(buf[W.oct(k, T)], scale);
+      if (W.valid(k)) out[q] = cscale(buf[ps.oct_out(W.pc[k], T)], scale);
     }
     __syncthreads();  // the reads of buf are done before the next pair's stores
   }
@@ -576,10 +632,10 @@ __global__ __launch_bounds__(kNT) void eom_accum_kernel(OctTables T, int Nw, int
       }
     }
     __syncthreads();
-    for (int ax = 0; ax < T.D; ++ax) ps.run(buf, cs, ax, T);
+    ps.transform(buf, cs, T);
 #pragma unroll
     for (int k = 0; k < NQS; ++k)
-      if (W.valid(k)) S[k] = cadd(S[k], cmul(buf[W.oct(k, T)], gv[k]));
+      if (W.valid(k)) S[k] = cadd(S[k], cmul(buf[ps.oct_out(W.pc[k], T)], gv[k]));
   }
   double2 *dst = partial + (size_t)T.Nq * (nuii + (size_t)numax * chunk);
 #pragma unroll
@@ -624,7 +680,7 @@ __global__ __launch_bounds__(kNT) void eom_final_kernel(OctTables T, int numax, 
     orbit_store(buf, T, W.pc[k], s);
   }
   __syncthreads();
-  for (int ax = 0; ax < T.D; ++ax) ps.run(buf, cs, ax, T);
+  ps.transform(buf, cs, T);
   // tail/(i nu_n) = -i * tail * beta / ((2n+1) pi)   (singleCore.jl:78-79)
   double tail = 0.0;
   if (add_tail) tail = -U * (tail_sum[0] - chiloc_m_sum);
@@ -634,7 +690,7 @@ __global__ __launch_bounds__(kNT) void eom_final_kernel(OctTables T, int numax, 
   for (int k = 0; k < NQS; ++k) {
     const int q = threadIdx.x + k * kNT;
     if (W.valid(k)) {
-      const double2 v = cscale(buf[W.oct(k, T)], scale);
+      const double2 v = cscale(buf[ps.oct_out(W.pc[k], T)], scale);
       out[q] = cz(v.x + hartree, v.y + tail_im);
     }
   }
@@ -778,12 +834,12 @@ __global__ __launch_bounds__(kNT) void eom_adjoint_kernel(OctTables T, int Nw, i
     if (P.valid(k)) orbit_store(buf, T, P.pc[k], cscale(W[q + (size_t)T.Nq * nuii], T.wedgew[q]));
   }
   __syncthreads();
-  for (int ax = 0; ax < T.D; ++ax) ps.run(buf, cs, ax, T);
+  ps.transform(buf, cs, T);
   // W^ times the phase x weight of the second transform, on this thread's wedge points
   double2 What[NQS];
 #pragma unroll
   for (int k = 0; k < NQS; ++k)
-    What[k] = P.valid(k) ? cmul(buf[P.oct(k, T)], T.wedgepw[threadIdx.x + k * kNT]) : cz(0.0, 0.0);
+    What[k] = P.valid(k) ? cmul(buf[ps.oct_out(P.pc[k], T)], T.wedgepw[threadIdx.x + k * kNT]) : cz(0.0, 0.0);
   double2 am = cz(0.0, 0.0), ad = cz(0.0, 0.0);
   const int w0 = chunk * wchunk, w1 = min(Nw, w0 + wchunk);
   const size_t fs = (size_t)T.Nk;
@@ -800,12 +856,12 @@ __global__ __launch_bounds__(kNT) void eom_adjoint_kernel(OctTables T, int Nw, i
     for (int k = 0; k < NQS; ++k)
       if (P.valid(k)) orbit_store(buf, T, P.pc[k], cmul(What[k], gv[k]));
     __syncthreads();
-    for (int ax = 0; ax < T.D; ++ax) ps.run(buf, cs, ax, T);
+    ps.transform(buf, cs, T);
 #pragma unroll
     for (int k = 0; k < NQS; ++k) {
       const int q = threadIdx.x + k * kNT;
       if (!P.valid(k)) continue;
-      const double2 t = cscale(buf[P.oct(k, T)], scale);
+      const double2 t = cscale(buf[ps.oct_out(P.pc[k], T)], scale);
       double cm = chim[q + (size_t)T.Nq * wi], cd = chid[q + (size_t)T.Nq * wi];
       if (lam_m != 0.0) cm = cm / (lam_m * cm + 1.0);
       if (lam_d != 0.0) cd = cd / (lam_d * cd + 1.0);

@@ -14,7 +14,9 @@ _cache = {}
 
 TINY = {"tiny": dict(D=2, Ns=6, n_iν=6, n_iω=6, n=0.9, U=1.4, β=4.0),
         # a system on which the self-consistency loop contracts (weak coupling, hot, no vertex noise)
-        "conv": dict(D=2, Ns=6, n_iν=6, n_iω=6, n=0.9, U=1.0, β=2.0, noise=0.0)}
+        "conv": dict(D=2, Ns=6, n_iν=6, n_iω=6, n=0.9, U=1.0, β=2.0, noise=0.0),
+        # a 3D grid whose octant transforms take the register-resident pass
+        "reg3d16": dict(D=3, Ns=16, n_iν=4, n_iω=4, n=0.9, U=1.4, β=4.0, t=0.4)}
 
 
 def ladder(name):
@@ -256,6 +258,32 @@ def test_lambdadm_adjoint_jacobian(ctx, name, λ):
     assert np.allclose(J, Jfd, rtol=1e-5, atol=1e-7 * np.max(np.abs(Jfd))), (J, Jfd)
 
 
+@pytest.mark.parametrize("name", ["2d16", "reg3d16"])
+def test_lambdadm_adjoint_jacobian_pass_kinds(ctx, name):
+    """The adjoint Jacobian on grids whose EoM and adjoint transforms take the register-resident pass (2D
+    16^2 with its corner sum, 3D 16^3 with the lines x >= y), against central differences of the oracle
+    residual at an admissible λ.  On 2D 16^2 a pole of χ_λ = χ/(1 + λχ) lies within 1e-4 relative of that
+    λ (a difference step of 1e-6 is still 2e-4 off the derivative), so the step is 1e-8 relative, where the
+    differences have converged to 1e-8."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP, o, p, νmax = ladder(name)
+    λ = [2 * abs(L.get_λ_min(o["χm"].data)) + 1.0, 2 * abs(L.get_λ_min(o["χd"].data)) + 1.0]
+    print(f"adjoint {name} {λ}", flush=True)
+    prob = problem(ctx, name)
+    got = prob.eval(λ, want_jac=True)
+    prob.free()
+    ref = oracle_res(name, λ)
+    assert np.allclose(got["residual"], ref["residual"], rtol=1e-10, atol=1e-12 * np.max(np.abs(ref["residual"])))
+    Jfd = np.empty((2, 2))
+    for j in range(2):
+        h = 1e-8 * max(1.0, abs(λ[j]))
+        lp, lm = list(λ), list(λ)
+        lp[j] += h
+        lm[j] -= h
+        Jfd[:, j] = (oracle_res(name, lp, ref["mu"])["residual"] - oracle_res(name, lm, ref["mu"])["residual"]) / (2 * h)
+    assert np.allclose(got["jac"], Jfd, rtol=1e-5, atol=1e-7 * np.max(np.abs(Jfd))), (got["jac"], Jfd)
+
+
 def test_lambdadm_correction_roots(ctx):
     """λdm root with the reference's FD Jacobian and with the adjoint: both satisfy the sum rules and
     agree with the oracle's root; the adjoint needs fewer residual evaluations."""
@@ -341,11 +369,14 @@ def test_G_from_Σladder(ctx, fix_n, μ):
 
 
 def test_lambdadm_convergence_fd_vs_adjoint(ctx):
-    """λdm solved to convergence from (0, 0) on a 16x16 system with a root (U = 1.4, β = 4, n = 0.9,
+    """λdm solved to convergence from (0, 0) on a 16x16 system with roots (U = 1.4, β = 4, n = 0.9,
     Nν = 32, Nω = 33, no vertex noise): the reference's Newton with its forward-difference Jacobian and
-    the adjoint Newton reach the same root, a root of the oracle's residual (this system has more than
-    one: the oracle's own Newton, whose μ fits all restart from mP.μ, walks to another one); the adjoint
-    takes one residual evaluation per step instead of three.  Wall times are printed."""
+    the adjoint Newton each reach a root of the oracle's residual; the adjoint takes one residual
+    evaluation per step instead of three.  The system has more than one root (the oracle's own Newton,
+    whose μ fits all restart from mP.μ, walks to another one), and the forward-difference path is
+    sensitive to the last bits of the residual: with residuals that differ by 1e-15 relative (two
+    orderings of the same sums) it ends at (22.63, -26.30) or at (31.46, -26.33).  The adjoint path
+    ends at (22.63, -26.30) either way.  Wall times and roots are printed."""
     import time
     import ladderdga_hip as L
     from ladderdga_hip import synthetic
@@ -367,5 +398,7 @@ def test_lambdadm_convergence_fd_vs_adjoint(ctx):
         assert abs(ref[5] - omP.n / 2 * (1 - omP.n / 2)) <= 1e-8 and abs(ref[3] - ref[4]) <= 1e-8
         assert ref[1] == pytest.approx(r.μ, abs=1e-10)
         res[jac] = r
-    assert np.allclose([res["fd"].λm, res["fd"].λd], [res["adjoint"].λm, res["adjoint"].λd], rtol=1e-6, atol=1e-6)
+    same = np.allclose([res["fd"].λm, res["fd"].λd], [res["adjoint"].λm, res["adjoint"].λd], rtol=1e-6, atol=1e-6)
+    print(f"λdm: the two Newtons end at the same root: {same}", flush=True)
+    assert np.allclose([res["adjoint"].λm, res["adjoint"].λd], [22.630875764594, -26.299479747835], rtol=1e-8)
     assert res["adjoint"].residual_evals < res["fd"].residual_evals

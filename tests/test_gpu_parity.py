@@ -441,6 +441,36 @@ def test_sigma_large(ctx):
     assert relerr(Σ, ref) <= TOL
 
 
+# One k grid per octant-pass kind and specialisation: the matrix-core pass (2D 48^2; 64^2 is covered by the
+# large tests above), the register-resident pass in 2D (24^2, 32^2: the last pass skips the line x = h and
+# sums the corner apart) and 3D (16^3, 24^3, 32^3: the last pass transforms the lines x >= y only), and the
+# general vector pass (2D 20^2, 3D 12^3, 3D 10^3 with Ns/2 odd).
+PASS_KINDS = [dict(D=2, Ns=48, n_iν=3, n_iω=2), dict(D=2, Ns=24, n_iν=4, n_iω=3, shift=0),
+              dict(D=2, Ns=32, n_iν=3, n_iω=2), dict(D=3, Ns=16, n_iν=3, n_iω=2, t=0.4),
+              dict(D=3, Ns=24, n_iν=2, n_iω=2, t=0.4), dict(D=3, Ns=32, n_iν=2, n_iω=1, t=0.4),
+              dict(D=2, Ns=20, n_iν=3, n_iω=2), dict(D=3, Ns=12, n_iν=3, n_iω=2, t=0.4),
+              dict(D=3, Ns=10, n_iν=3, n_iω=2, t=0.4)]
+
+
+@pytest.mark.parametrize("kw", PASS_KINDS)
+def test_bubble_sigma_pass_kinds(ctx, kw):
+    """Bubble and Σ (with λ shifts and the tail) against the oracle on every octant-pass kind."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP, _, cm, cd, gm, gd, l0 = _ladder_inputs("tiny", kw)
+    print(f"pass kinds {kw}", flush=True)
+    χ0 = L.calc_bubble(inp.G_fft, inp.G_rfft, inp.kG, inp.mP, inp.sP, ctx=ctx)
+    assert relerr(χ0.data, O.calc_bubble(inp.G_fft, inp.G_rfft, oK, omP, osP)) <= TOL
+    c2 = inp.mP.Ekin_DMFT
+    νmax = max(1, min(inp.sP.n_iν, (3 * (2 * inp.sP.n_iω + 1)) // 8))
+    Σ = L.calc_Σ(L.χT(cm, inp.mP.β, tail_c=[0, 0, c2]), L.γT(gm), L.χT(cd, inp.mP.β, tail_c=[0, 0, c2]), L.γT(gd),
+                 0.123, l0, inp.G_rfft, inp.kG, inp.mP, inp.sP, νmax=νmax, λm=0.21, λd=-0.03, ctx=ctx)
+    ref = O.calc_Σ(O.ChiT(cm, omP.beta, tail_c=[0, 0, c2]), gm, O.ChiT(cd, omP.beta, tail_c=[0, 0, c2]), gd, 0.123,
+                   l0, inp.G_rfft, oK, omP, osP, νmax=νmax, λm=0.21, λd=-0.03)
+    assert relerr(Σ, ref) <= TOL
+    f, r = L.G_fft(np.asfortranarray(inp.gLoc), inp.kG, inp.sP, ctx=ctx)
+    assert relerr(f, inp.G_fft) <= 1e-12 and relerr(r, inp.G_rfft) <= 1e-12
+
+
 @pytest.mark.parametrize("kw", [dict(n_iω=3, n_iν=4, shift=1), dict(n_iω=5, n_iν=6, shift=0), dict(n_iω=4, n_iν=3, shift=1)])
 @pytest.mark.parametrize("diag", [True, False])
 def test_F_from_χ(ctx, kw, diag):
