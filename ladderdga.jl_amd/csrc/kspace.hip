@@ -378,8 +378,12 @@ struct MfmaPass {
     }
   }
   __device__ __forceinline__ void run(double2 *buf, const double *, int ax, const OctTables &T) const {
+    run_lines(buf, ax, T, T.NO / (h + 1));
+  }
+  // pass ax over the lines 0 .. nlines-1 (the first nlines of the octant's line order)
+  __device__ __forceinline__ void run_lines(double2 *buf, int ax, const OctTables &T, int nlines) const {
     constexpr int n = h + 1;
-    const int stride = T.stride[ax], nlines = T.NO / n;
+    const int stride = T.stride[ax];
     const int nct = (nlines + 7) / 8;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j = lane & 15, k = lane >> 4, part = j & 1;
@@ -431,10 +435,33 @@ struct MfmaPass {
     }
     __syncthreads();
   }
+  // All D passes for outputs read at wedge points (oct_out).  In 2D the last
+  // pass skips the line x = h: the points (h, c1) are read at (c1, h) (the
+  // output is symmetric), and the corner (h, h) = sum_y (-1)^y T(h, y) of the
+  // first pass's output is summed apart by two lanes before the last pass (no
+  // tile of the last pass touches the line x = h).
   __device__ __forceinline__ void transform(double2 *buf, const double *cs, const OctTables &T) const {
-    for (int ax = 0; ax < T.D; ++ax) run(buf, cs, ax, T);
+    if (T.D != 2) {
+      for (int ax = 0; ax < T.D; ++ax) run(buf, cs, ax, T);
+      return;
+    }
+    run(buf, cs, 0, T);
+    constexpr int n = h + 1;
+    if (threadIdx.x < 2) {  // read before the last pass: it leaves the line x = h alone
+      double *bd = reinterpret_cast<double *>(buf);
+      const int part = threadIdx.x;
+      double s = 0.0;
+#pragma unroll
+      for (int y = 0; y <= h; ++y) s += (y & 1) ? -bd[2 * (h + n * y) + part] : bd[2 * (h + n * y) + part];
+      bd[2 * (h + n * h) + part] = s;
+    }
+    run_lines(buf, 1, T, h);
   }
-  __device__ __forceinline__ int oct_out(int pc, const OctTables &T) const { return oct_of(pc, T); }
+  __device__ __forceinline__ int oct_out(int pc, const OctTables &T) const {
+    const int a = pc & 0xff, b = (pc >> 8) & 0xff;
+    if (T.D == 2) return a == h ? b + h * (h + 1) : a + (h + 1) * b;
+    return oct_of(pc, T);
+  }
 };
 
 
