@@ -635,34 +635,56 @@ __global__ __launch_bounds__(kNT) void eom_accum_kernel(OctTables T, int Nw, int
   }
   const int w0 = chunk * wchunk, w1 = min(Nw, w0 + wchunk);
   const size_t fs = (size_t)T.Nk;
-  for (int wi = w0; wi < w1; ++wi) {
+  // nu-window of gamma at w: nu0 index (1-based) = n_inu + shift*trunc(wn/2) + 1  (helpers.jl:58); the
+  // 0-based index of nu_n = nuii, skipped past the end of the nu list (singleCore.jl:59-60)
+  auto nui_of = [&](int wi) { return n_inu + shift * trunc_half(wi - n_iw) + nuii; };
+  auto next_w = [&](int wi) {
+    while (wi < w1 && nui_of(wi) > 2 * n_inu - 1) ++wi;
+    return wi;
+  };
+  // the inputs of K at this thread's wedge points, loaded one omega ahead: the
+  // passes of one omega hide the loads of the next
+  double km_[NQS], kd_[NQS];
+  double2 kgm[NQS], kgd[NQS], kl0[NQS];
+  auto load_k = [&](int wi) {
+    const size_t col = (size_t)T.Nq * (nui_of(wi) + (size_t)Nnu * wi);
+#pragma unroll
+    for (int k = 0; k < NQS; ++k) {
+      const int q = W.valid(k) ? threadIdx.x + k * kNT : 0;
+      km_[k] = chim[q + (size_t)T.Nq * wi];
+      kd_[k] = chid[q + (size_t)T.Nq * wi];
+      kgm[k] = gm[q + col];
+      kgd[k] = gd[q + col];
+      kl0[k] = l0[q + col];
+    }
+  };
+  int wi = next_w(w0);
+  if (wi < w1) load_k(wi);
+  while (wi < w1) {
     const int wn = wi - n_iw;
-    // nu-window of gamma at this w: nu0 index (1-based) = n_inu + shift*trunc(wn/2) + 1  (helpers.jl:58)
-    const int nui = n_inu + shift * trunc_half(wn) + nuii;  // 0-based index of nu_n = nuii
-    if (nui > 2 * n_inu - 1) continue;                     // nu list ends at 2 n_inu (singleCore.jl:59-60)
-    // G(nu + w) at this thread's points, loaded first: the K loads and the passes hide the latency
+    // G(nu + w) at this thread's points, loaded first: the passes hide the latency
     const double2 *g = Grfft + (size_t)(nuii + wn - fft_first) * fs;
     double2 gv[NQS];
 #pragma unroll
     for (int k = 0; k < NQS; ++k) gv[k] = g[f[k]];
     __syncthreads();  // the previous omega's reads of buf are done
-    // K on the wedge (q is the fastest index of gamma, lambda0 and chi: coalesced)
 #pragma unroll
     for (int k = 0; k < NQS; ++k) {
-      const int q = threadIdx.x + k * kNT;
       if (W.valid(k)) {
-        double cm = chim[q + (size_t)T.Nq * wi], cd = chid[q + (size_t)T.Nq * wi];
+        double cm = km_[k], cd = kd_[k];
         if (lam_m != 0.0) cm = cm / (lam_m * cm + 1.0);
         if (lam_d != 0.0) cd = cd / (lam_d * cd + 1.0);
-        const size_t idx = q + (size_t)T.Nq * (nui + (size_t)Nnu * wi);
-        orbit_store(buf, T, W.pc[k], cscale(eom_value(kind, U, gm[idx], gd[idx], cm, cd, l0[idx]), wq[k]));
+        orbit_store(buf, T, W.pc[k], cscale(eom_value(kind, U, kgm[k], kgd[k], cm, cd, kl0[k]), wq[k]));
       }
     }
+    const int wnext = next_w(wi + 1);
+    if (wnext < w1) load_k(wnext);
     __syncthreads();
     ps.transform(buf, cs, T);
 #pragma unroll
     for (int k = 0; k < NQS; ++k)
       if (W.valid(k)) S[k] = cadd(S[k], cmul(buf[ps.oct_out(W.pc[k], T)], gv[k]));
+    wi = wnext;
   }
   double2 *dst = partial + (size_t)T.Nq * (nuii + (size_t)numax * chunk);
 #pragma unroll
