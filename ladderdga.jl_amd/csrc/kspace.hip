@@ -508,6 +508,20 @@ __device__ __forceinline__ void orbit_store(double2 *buf, const OctTables &T, in
 }
 
 // ---------------------------------------------------------------------------
+// The G tables on the wedge: dst[q + Nq f] = src[full index of wedge point q +
+// Nk f] for the planes f of fft_range.  The octant kernels gather the wedge
+// points of a plane for every (nu, w); on the full grid each such gather reads
+// a cache line per point (the wedge is 1/2^D/D! of the grid), on the wedge it
+// is one contiguous Nq-vector that stays in L2.
+// ---------------------------------------------------------------------------
+__global__ void wedge_tables_kernel(OctTables T, int nplanes, const double2 *__restrict__ src,
+                                    double2 *__restrict__ dst) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x, f = blockIdx.y;
+  if (q >= T.Nq || f >= nplanes) return;
+  dst[q + (size_t)T.Nq * f] = src[T.wedgefull[q] + (size_t)T.Nk * f];
+}
+
+// ---------------------------------------------------------------------------
 // bubble: per (nu, w) pair the product of the two G tables on the octant, the
 // octant transform on all D axes, then the wedge points read from it.  A
 // workgroup takes a contiguous range of pairs (w-major, nu fastest) and gathers
@@ -535,10 +549,10 @@ __global__ __launch_bounds__(kNT) void bubble_kernel(OctTables T, int Nnu, int n
 #pragma unroll
   for (int k = 0; k < NQS; ++k) {
     const int q = threadIdx.x + k * kNT;
-    f[k] = W.valid(k) ? T.wedgefull[q] : 0;
+    f[k] = W.valid(k) ? q : 0;
     pw[k] = W.valid(k) ? T.wedgepw[q] : cz(0.0, 0.0);
   }
-  const size_t fs = (size_t)T.Nk;
+  const size_t fs = (size_t)T.Nq;  // the G tables on the wedge, one plane per frequency
   double2 va[NQS], vb[NQS];
   auto gather = [&](int P) {
     const int nui = P % Nnu, wn = P / Nnu - n_iw;
@@ -629,12 +643,12 @@ __global__ __launch_bounds__(kNT) void eom_accum_kernel(OctTables T, int Nw, int
 #pragma unroll
   for (int k = 0; k < NQS; ++k) {
     const int q = threadIdx.x + k * kNT;
-    f[k] = W.valid(k) ? T.wedgefull[q] : 0;
+    f[k] = W.valid(k) ? q : 0;
     wq[k] = W.valid(k) ? T.wedgew[q] : 0.0;
     S[k] = cz(0.0, 0.0);
   }
   const int w0 = chunk * wchunk, w1 = min(Nw, w0 + wchunk);
-  const size_t fs = (size_t)T.Nk;
+  const size_t fs = (size_t)T.Nq;  // the G tables on the wedge, one plane per frequency
   // nu-window of gamma at w: nu0 index (1-based) = n_inu + shift*trunc(wn/2) + 1  (helpers.jl:58); the
   // 0-based index of nu_n = nuii, skipped past the end of the nu list (singleCore.jl:59-60)
   auto nui_of = [&](int wi) { return n_inu + shift * trunc_half(wi - n_iw) + nuii; };
@@ -875,7 +889,7 @@ __global__ __launch_bounds__(kNT) void eom_adjoint_kernel(OctTables T, int Nw, i
   P.load(T);
   int f[NQS];
 #pragma unroll
-  for (int k = 0; k < NQS; ++k) f[k] = P.valid(k) ? T.wedgefull[threadIdx.x + k * kNT] : 0;
+  for (int k = 0; k < NQS; ++k) f[k] = P.valid(k) ? threadIdx.x + k * kNT : 0;
   // forward transform of W(., nu) (weights w(q) of the octant fold)
 #pragma unroll
   for (int k = 0; k < NQS; ++k) {
@@ -891,7 +905,7 @@ __global__ __launch_bounds__(kNT) void eom_adjoint_kernel(OctTables T, int Nw, i
     What[k] = P.valid(k) ? cmul(buf[ps.oct_out(P.pc[k], T)], T.wedgepw[threadIdx.x + k * kNT]) : cz(0.0, 0.0);
   double2 am = cz(0.0, 0.0), ad = cz(0.0, 0.0);
   const int w0 = chunk * wchunk, w1 = min(Nw, w0 + wchunk);
-  const size_t fs = (size_t)T.Nk;
+  const size_t fs = (size_t)T.Nq;  // the G tables on the wedge, one plane per frequency
   for (int wi = w0; wi < w1; ++wi) {
     const int wn = wi - n_iw;
     const int nui = n_inu + shift * trunc_half(wn) + nuii;
@@ -1082,6 +1096,20 @@ static int with_cfg(ldga_ctx *ctx, F &&f) {
   return with_nqs(ctx, [&](auto c) { return f(c, ValuPass{}); });
 }
 
+// the full-grid G table src (Nk x n_fft) on the wedge, in the context's scratch of the given role
+static const double2 *wedge_tables(ldga_ctx *ctx, const ldga_freq *fr, const double2 *src, const char *role,
+                                   hipStream_t st) {
+  const OctTables &T = ctx->oct;
+  double2 *dst = (double2 *)ldga_scratch(ctx, role, (size_t)T.Nq * fr->n_fft * 16, st);
+  if (!dst) return nullptr;
+  wedge_tables_kernel<<<dim3((T.Nq + 255) / 256, fr->n_fft), 256, 0, st>>>(T, fr->n_fft, src, dst);
+  if (hipGetLastError() != hipSuccess) {
+    ldga_fail(ctx, LDGA_ERR_HIP, "wedge_tables_kernel launch failed");
+    return nullptr;
+  }
+  return dst;
+}
+
 int launch_bubble(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double2 *Gfft, const double2 *Grfft,
                   double2 *chi0, hipStream_t st) {
   const OctTables &T = ctx->oct;
@@ -1099,13 +1127,15 @@ int launch_bubble(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double2
                      fr->fft_first + fr->n_fft - 1, lo, hi);
   const size_t smem = (size_t)T.NO * 16 + (size_t)T.Ns * 8;
   const double scale = -beta / (T.Nk * T.Nk);
+  const double2 *Gw = wedge_tables(ctx, fr, Gfft, "bubble_gw", st), *Grw = wedge_tables(ctx, fr, Grfft, "bubble_grw", st);
+  if (!Gw || !Grw) return LDGA_ERR_HIP;
   // one round of resident workgroups, each over a contiguous range of pairs
   return with_cfg(ctx, [&](auto c, auto ps) {
     auto kern = bubble_kernel<decltype(c)::value, decltype(ps)>;
     const int npairs = Nnu * Nw, res = resident_wgs(ctx, (const void *)kern, smem);
     const int per_wg = (npairs + res - 1) / res;
     return oct_launch(ctx, kern, (npairs + per_wg - 1) / per_wg, smem, st, T, Nnu, ntot, fr->n_iw, fr->shift,
-                      fr->fft_first, per_wg, Gfft, Grfft, chi0, scale);
+                      fr->fft_first, per_wg, Gw, Grw, chi0, scale);
   });
 }
 
@@ -1244,10 +1274,12 @@ int launch_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int n
     if (rc) return rc;
   }
   const size_t smem = (size_t)T.NO * 16 + (size_t)T.Ns * 8;
+  const double2 *Grw = wedge_tables(ctx, fr, Grfft, "eom_grw", st);
+  if (!Grw) return LDGA_ERR_HIP;
   int rc = with_cfg(ctx, [&](auto c, auto ps) {
     return oct_launch(ctx, eom_accum_kernel<decltype(c)::value, decltype(ps)>, numax * nchunks, smem, st, T, Nw, Nnu, fr->n_inu,
                       fr->n_iw, fr->shift, fr->fft_first, numax, wchunk, eom_kind, mp->U, lam_m, lam_d, chi_m, chi_d,
-                      gamma_m, gamma_d, lambda0, Grfft, partial);
+                      gamma_m, gamma_d, lambda0, Grw, partial);
   });
   if (rc) return rc;
   const double hartree = (eom_kind == LDGA_EOM_FULL || eom_kind == LDGA_EOM_REST) ? mp->n * mp->U / 2.0 : 0.0;
@@ -1271,10 +1303,12 @@ int launch_eom_adjoint(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp,
   if (!partial) return LDGA_ERR_HIP;
   const size_t smem = (size_t)T.NO * 16 + (size_t)T.Ns * 8;
   const double scale = 1.0 / (T.Nk * T.Nk);
+  const double2 *Grw = wedge_tables(ctx, fr, Grfft, "eom_grw", st);
+  if (!Grw) return LDGA_ERR_HIP;
   const int rc = with_cfg(ctx, [&](auto c, auto ps) {
     return oct_launch(ctx, eom_adjoint_kernel<decltype(c)::value, decltype(ps)>, numax * nchunks, smem, st, T, Nw, Nnu, fr->n_inu,
                       fr->n_iw, fr->shift, fr->fft_first, numax, wchunk, mp->U, lam_m, lam_d, chi_m, chi_d, gamma_m,
-                      gamma_d, Grfft, W, scale, partial);
+                      gamma_d, Grw, W, scale, partial);
   });
   if (rc) return rc;
   sum_pairs_kernel<<<1, 256, 0, st>>>(numax * nchunks, partial, out_dev);
