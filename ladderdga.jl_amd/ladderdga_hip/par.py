@@ -13,13 +13,15 @@ Names and argument meaning follow the reference; Julia's `calc_Σ_par!`,
 (`calc_Σ_par!` becomes `calc_Σ_par_inplace`), and `collect_χ₀` is spelled
 `collect_χ0` (a subscript digit is no Python identifier character).
 
-Differences kept on purpose:
+Differences kept on purpose (by default; set_reference_semantics(True)
+switches both to the reference's worker code as written):
 - calc_χγ_par solves the plain BSE, as calc_χγ does.  The reference's worker
   kernel (ladderDGATools.jl:92-95) requires the asymptotic helper of BSE_SC
   and throws without it; that helper is not part of this library.
 - calc_Σ_par with tc=False divides the ω sum by β, as calc_Σ does.  The
   reference's worker EoM (ladderDGATools.jl:189) applies 1/β only together
-  with the tail term, so its tc=false result is β times larger.
+  with the tail term, so its tc=false result is β times larger (the Hartree
+  term, added when collecting (ladderDGATools.jl:201-215), is not scaled).
 """
 from __future__ import annotations
 
@@ -56,6 +58,7 @@ class WorkerCache:
         self.χloc_m_sum = 0.0
         self.EoMVars_initialized = False
         self.EoM_νGrid = range(0, 0)
+        self.reference_semantics = getattr(self, "reference_semantics", False)
 
     def put(self, name: str, host: np.ndarray | None = None, shape=None, dtype=None):
         """Store an array on the device under `name` (upload `host`, or allocate `shape`)."""
@@ -115,6 +118,13 @@ def update_wcaches_G_rfft(G):
     update_wcache("G_fft_reverse", G)
 
 
+def set_reference_semantics(on: bool = True):
+    """Follow the reference's worker code where it differs from the serial functions: calc_χγ_par raises
+    as bse_inv does without the BSE_SC helper (ladderDGATools.jl:92), and calc_Σ_par(tc=False) returns
+    the ω sum without the 1/β (ladderDGATools.jl:189).  Off by default; kept across clear_wcache."""
+    wcache.reference_semantics = bool(on)
+
+
 def clear_wcache():
     """clear_wcache!() (parallelization_helpers.jl:313-324): frees every device buffer."""
     wcache.clear()
@@ -158,6 +168,8 @@ def calc_χγ_par(type_: str, Γr, kG: KGrid, mP: ModelParameters, sP: Simulatio
     s = {"m": 1, "d": -1}.get(t)
     if s is None:
         raise ValueError("Unkown type")
+    if wcache.reference_semantics:
+        raise RuntimeError("Current version ONLY supports BSE_Asym_Helper!")
     ctx = wcache.context()
     χ0 = wcache.get("χ₀")
     Nν, Nω = 2 * sP.n_iν, 2 * sP.n_iω + 1
@@ -260,6 +272,9 @@ def calc_Σ_par_inplace(Σ_ladder, λm: float = 0.0, λd: float = 0.0, collect_d
     if not collect_data:
         return None
     Σ = g("Σ_ladder").to_host()
+    if wcache.reference_semantics and not tc:
+        hartree = mP.n * mP.U / 2.0
+        Σ = (Σ - hartree) * mP.β + hartree
     if Σ_ladder is None:
         return Σ
     if Σ_ladder.shape != Σ.shape:
@@ -268,6 +283,7 @@ def calc_Σ_par_inplace(Σ_ladder, λm: float = 0.0, λd: float = 0.0, collect_d
     return Σ_ladder
 
 
-__all__ = ["wcache", "WorkerCache", "update_wcache", "update_wcaches_G_rfft", "clear_wcache", "calc_bubble_par",
+__all__ = ["wcache", "WorkerCache", "update_wcache", "update_wcaches_G_rfft", "clear_wcache", "set_reference_semantics",
+           "calc_bubble_par",
            "collect_χ0", "calc_χγ_par", "collect_χ", "collect_γ", "initialize_EoM", "calc_Σ_par",
            "calc_Σ_par_inplace"]

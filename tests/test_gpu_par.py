@@ -99,6 +99,42 @@ def test_par_api_matches_serial_and_oracle(ctx, name, kw):
         L.calc_Σ_par()
 
 
+def test_par_api_reference_semantics(ctx):
+    """set_reference_semantics(True): calc_χγ_par raises as the reference's worker kernel does without the
+    BSE_SC helper (ladderDGATools.jl:92), and calc_Σ_par(tc=False) returns the ω sum without the 1/β
+    (ladderDGATools.jl:189), the Hartree term of collect_Σ! (:201-215) unscaled: β (Σ - H) + H of the
+    default result, checked against the oracle's sum.  tc=True is unchanged."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP = _inputs("tiny", dict(D=2, Ns=6, n_iν=5, n_iω=4, n=0.9, U=1.4))
+    kG, mP, sP = inp.kG, inp.mP, inp.sP
+    L.clear_wcache()
+    for k, v in (("kG", kG), ("mP", mP), ("sP", sP), ("G_fft", inp.G_fft), ("G_fft_reverse", inp.G_rfft)):
+        L.update_wcache(k, v, ctx=ctx)
+    bubble = L.calc_bubble_par(kG, mP, sP)
+    (χm, γm), (χd, γd) = L.calc_χγ_par("m", inp.Γ_m, kG, mP, sP), L.calc_χγ_par("d", inp.Γ_d, kG, mP, sP)
+    with pytest.warns(UserWarning):
+        λ0 = L.calc_λ0(bubble, inp.F_m, mP, sP, kG, ctx=ctx)
+    νmax = sP.n_iν
+    L.initialize_EoM(inp.G_rfft, inp.χloc_m_sum, λ0, range(0, νmax), kG, mP, sP, χ_m=χm, γ_m=γm, χ_d=χd, γ_d=γd,
+                     force_reinit=True)
+    default = {tc: L.calc_Σ_par(λm=0.5, λd=0.25, tc=tc) for tc in (False, True)}
+    try:
+        L.set_reference_semantics(True)
+        with pytest.raises(RuntimeError, match="BSE_Asym_Helper"):
+            L.calc_χγ_par("m", inp.Γ_m, kG, mP, sP)
+        ref = {tc: L.calc_Σ_par(λm=0.5, λd=0.25, tc=tc) for tc in (False, True)}
+    finally:
+        L.set_reference_semantics(False)
+    H = mP.n * mP.U / 2.0
+    assert relerr(ref[True], default[True]) == 0.0
+    assert relerr(ref[False], (default[False] - H) * mP.β + H) <= 1e-14
+    o = O.calc_Σ(O.ChiT(χm.data, omP.beta, tail_c=list(χm.tail_c)), γm.data,
+                 O.ChiT(χd.data, omP.beta, tail_c=list(χd.tail_c)), γd.data, inp.χloc_m_sum, λ0, inp.G_rfft, oK, omP,
+                 osP, νmax=νmax, λm=0.5, λd=0.25, tc=False)
+    assert relerr(ref[False] - H, (o - omP.n * omP.U / 2.0) * omP.beta) <= TOL
+    L.clear_wcache()
+
+
 def test_par_api_rejects_partial_usable_range(ctx):
     """initialize_EoM refuses χ whose usable ω range is not the whole interval, as the reference's does
     (parallelization_helpers.jl:242-244)."""
