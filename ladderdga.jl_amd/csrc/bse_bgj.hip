@@ -43,7 +43,7 @@
 //
 // Refinement: r = 1 - A^T y with Gamma and chi0 in FP64 (Gamma is L2-resident:
 // all q of one w share it), y += X r, until |X r|_inf <= 2^-50 |y|_inf, or the
-// next correction extrapolated from the contraction rate is <= 2^-48 |y|_inf,
+// next correction extrapolated from the contraction rate is <= 2^-42 |y|_inf,
 // or the correction stops halving (the FP64 residual's rounding level).
 #include <algorithm>
 #include <cstring>
@@ -498,7 +498,7 @@ __device__ __forceinline__ void bgj_solve(BgjShared<NT> &sh, const int lb, const
       ym = fmaxf(ym, sh.redf[it & 1][1][k]);
     }
     const double ratio = (double)dm / (double)ym;
-    const bool converged = ratio <= 0x1p-50 || (it > 0 && ratio * ratio <= 0x1p-48 * last_ratio);
+    const bool converged = ratio <= 0x1p-50 || (it > 0 && ratio * ratio <= 0x1p-42 * last_ratio);
     const bool stagnant = it > 0 && !(ratio < 0.5 * last_ratio);
     const bool stop = converged || stagnant;
     last_ratio = ratio;
