@@ -725,23 +725,33 @@ __global__ __launch_bounds__(kNT) void eom_final_kernel(OctTables T, int numax, 
   ps.init(T);
   WedgeSlots<NQS> W;
   W.load(T);
-  // the chunk partials of this nu, four chunks in flight per step
+  // the chunk partials of this nu, summed in chunk order: eight chunks of
+  // every wedge slot in flight per round (one memory round trip per eight
+  // chunks, not per chunk and slot)
   const size_t cstride = (size_t)T.Nq * numax;
   const double2 *pp = partial + (size_t)T.Nq * nuii;
+  constexpr int CB = 8;
+  int f[NQS];
+  double2 s[NQS];
 #pragma unroll
   for (int k = 0; k < NQS; ++k) {
-    const int q = threadIdx.x + k * kNT;
-    if (!W.valid(k)) continue;
-    double2 s = cz(0.0, 0.0);
-    int c = 0;
-    for (; c + 4 <= nchunks; c += 4) {
-      const double2 a0 = pp[cstride * c + q], a1 = pp[cstride * (c + 1) + q];
-      const double2 a2 = pp[cstride * (c + 2) + q], a3 = pp[cstride * (c + 3) + q];
-      s = cadd(s, cadd(cadd(a0, a1), cadd(a2, a3)));
-    }
-    for (; c < nchunks; ++c) s = cadd(s, pp[cstride * c + q]);
-    orbit_store(buf, T, W.pc[k], s);
+    f[k] = W.valid(k) ? threadIdx.x + k * kNT : 0;
+    s[k] = cz(0.0, 0.0);
   }
+  for (int c0 = 0; c0 < nchunks; c0 += CB) {
+    double2 a[NQS][CB];
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+#pragma unroll
+      for (int k = 0; k < NQS; ++k) a[k][c] = c0 + c < nchunks ? pp[cstride * (c0 + c) + f[k]] : cz(0.0, 0.0);
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+#pragma unroll
+      for (int k = 0; k < NQS; ++k) s[k] = cadd(s[k], a[k][c]);
+  }
+#pragma unroll
+  for (int k = 0; k < NQS; ++k)
+    if (W.valid(k)) orbit_store(buf, T, W.pc[k], s[k]);
   __syncthreads();
   ps.transform(buf, cs, T);
   // tail/(i nu_n) = -i * tail * beta / ((2n+1) pi)   (singleCore.jl:78-79)
@@ -762,13 +772,22 @@ __global__ __launch_bounds__(kNT) void eom_final_kernel(OctTables T, int numax, 
 // ---------------------------------------------------------------------------
 // lambda0 (plain sums): fsum[nu',w] = sum_nu F[nu,nu',w]; lambda0 = chi0 fsum / beta^2
 // ---------------------------------------------------------------------------
-__global__ void fsum_kernel(int Nnu, int Nw, const double2 *__restrict__ F, double2 *__restrict__ fsum) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;  // (nu', w)
+// One wave per column (nu', w): the lanes read the column's contiguous nu
+// entries (coalesced) and reduce them with shuffles.
+constexpr int kFsumNT = 256;
+__global__ __launch_bounds__(kFsumNT) void fsum_kernel(int Nnu, int Nw, const double2 *__restrict__ F,
+                                                       double2 *__restrict__ fsum) {
+  const int col = blockIdx.x * (kFsumNT / 64) + (threadIdx.x >> 6);  // (nu', w)
   if (col >= Nnu * Nw) return;
+  const int lane = threadIdx.x & 63;
   const double2 *c = F + (size_t)Nnu * col;
   double2 s = cz(0.0, 0.0);
-  for (int v = 0; v < Nnu; ++v) s = cadd(s, c[v]);
-  fsum[col] = s;
+  for (int v = lane; v < Nnu; v += 64) s = cadd(s, c[v]);
+  for (int off = 32; off > 0; off >>= 1) {
+    s.x += __shfl_xor(s.x, off);
+    s.y += __shfl_xor(s.y, off);
+  }
+  if (lane == 0) fsum[col] = s;
 }
 
 __global__ void lambda0_kernel(int Nq, int Nnu, int Nw, int shell, double inv_b2, const double2 *__restrict__ fsum,
@@ -785,31 +804,37 @@ __global__ void lambda0_kernel(int Nq, int Nnu, int Nw, int shell, double inv_b2
 
 // ---------------------------------------------------------------------------
 // sum_kw: a device-wide FP64 reduction over the usable (q, w) block; the last
-// workgroup adds the tail terms and writes the three sums
+// workgroup adds the tail terms (the w sum of the 1/(i w)^2 tail comes from the
+// host) and writes the three sums.  Up to kSumGrid workgroups, each thread with
+// kSumPer elements whose loads are all issued before the sums.
 // ---------------------------------------------------------------------------
 constexpr int kSumNT = 1024;  // the single-workgroup k integral below
-constexpr int kSumRB = 256, kSumGrid = 64;
+constexpr int kSumRB = 256, kSumGrid = 256, kSumPer = 4;
 __global__ __launch_bounds__(kSumRB) void sum_kw_kernel(const double *__restrict__ chi, const double *__restrict__ kmult,
                                                         int Nq, int w_lo, int w_hi, double Nk, double lam, double c2,
-                                                        double beta, int n_iw, double *part, unsigned *ticket,
+                                                        double beta, double tail, double *part, unsigned *ticket,
                                                         double *__restrict__ out) {
   double v[2] = {0.0, 0.0};
   const int nw = w_hi - w_lo + 1, total = Nq * nw, stride = gridDim.x * blockDim.x;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-    const int q = i % Nq;
-    const double x = chi[i + Nq * w_lo];  // chi[q + Nq w], w = w_lo + i / Nq
-    const double f = (lam != 0.0) ? x / (lam * x + 1.0) : x;
-    v[0] = fma(kmult[q], f, v[0]);
-    v[1] = fma(kmult[q], -f * f, v[1]);
+  for (int i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < total; i0 += kSumPer * stride) {
+    double x[kSumPer], km[kSumPer];
+#pragma unroll
+    for (int u = 0; u < kSumPer; ++u) {
+      const int i = i0 + u * stride;
+      const bool ok = i < total;
+      const int ic = ok ? i : 0;
+      x[u] = ok ? chi[ic + Nq * w_lo] : 0.0;  // chi[q + Nq w], w = w_lo + i / Nq
+      km[u] = ok ? kmult[ic - (ic / Nq) * Nq] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSumPer; ++u) {
+      const double f = (lam != 0.0) ? x[u] / (lam * x[u] + 1.0) : x[u];
+      v[0] = fma(km[u], f, v[0]);
+      v[1] = fma(km[u], -f * f, v[1]);
+    }
   }
   double tot[2];
   if (!ldga_red::device_sum<2>(v, part, ticket, tot) || threadIdx.x != 0) return;
-  // Re(c2/(i w_n)^2) = -c2 beta^2 / (4 pi^2 m^2), zero at m = 0  (DataTypes.jl:223-225)
-  double tail = 0.0;
-  for (int w = w_lo; w <= w_hi; ++w) {
-    const int m = w - n_iw;
-    if (m != 0) tail += -c2 * beta * beta / (4.0 * M_PI * M_PI * (double)m * (double)m);
-  }
   const double konst = -beta * beta / 12.0 * c2;
   out[0] = (tot[0] / Nk - tail + konst) / beta;
   out[1] = (tot[1] / Nk - tail + konst) / beta;
@@ -1153,7 +1178,7 @@ int launch_lambda0(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double
   const int Nnu = 2 * fr->n_inu, Nw = 2 * fr->n_iw + 1, Nq = ctx->Nq;
   double2 *fsum = (double2 *)ldga_scratch(ctx, "fsum", (size_t)Nnu * Nw * 16, st);
   if (!fsum) return LDGA_ERR_HIP;
-  fsum_kernel<<<(Nnu * Nw + 255) / 256, 256, 0, st>>>(Nnu, Nw, F, fsum);
+  fsum_kernel<<<(Nnu * Nw + kFsumNT / 64 - 1) / (kFsumNT / 64), kFsumNT, 0, st>>>(Nnu, Nw, F, fsum);
   const size_t total = (size_t)Nq * Nnu * Nw;
   lambda0_kernel<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(Nq, Nnu, Nw, fr->n_inu_shell, 1.0 / (beta * beta),
                                                                   fsum, chi0, lambda0);
@@ -1211,8 +1236,15 @@ int launch_sum_kw(ldga_ctx *ctx, const double *chi, int Nw, int w_lo, int w_hi, 
   if (!part || !ticket) return LDGA_ERR_HIP;
   const int total = T.Nq * (w_hi - w_lo + 1);
   const unsigned grid = (unsigned)std::min(kSumGrid, (total + kSumRB - 1) / kSumRB);
-  sum_kw_kernel<<<grid, kSumRB, 0, st>>>(chi, T.kmult, T.Nq, w_lo, w_hi, T.Nk, lam, c2, beta, (Nw - 1) / 2, part,
-                                         ticket, out_dev);
+  // Re(c2/(i w_n)^2) = -c2 beta^2 / (4 pi^2 m^2), zero at m = 0, summed over the slice  (DataTypes.jl:223-225)
+  const int n_iw = (Nw - 1) / 2;
+  double tail = 0.0;
+  for (int w = w_lo; w <= w_hi; ++w) {
+    const int m = w - n_iw;
+    if (m != 0) tail += -c2 * beta * beta / (4.0 * M_PI * M_PI * (double)m * (double)m);
+  }
+  sum_kw_kernel<<<grid, kSumRB, 0, st>>>(chi, T.kmult, T.Nq, w_lo, w_hi, T.Nk, lam, c2, beta, tail, part, ticket,
+                                         out_dev);
   LDGA_HIP_TRY(ctx, hipGetLastError());
   return LDGA_OK;
 }
