@@ -614,7 +614,10 @@ __device__ __forceinline__ double2 eom_value(int kind, double U, double2 gm, dou
 // chi is never written), times the fold weight, and stored on each point's
 // orbit; after the passes the product with G_rfft(nu + w) is accumulated on the
 // wedge points, and the phase x weight of the back transform applied once at
-// the end.
+// the end.  With tpart != NULL every workgroup L also sums kMult[q] chi_m^lam_m
+// over its share (elements L kNT + i + j gridDim.x kNT of the Nq x Nw block) into
+// tpart[L], for the 1/(i nu) tail of eom_final_kernel: the k-omega sum needs no
+// launch or device-wide fence of its own.
 template <int NQS, class PS>
 __global__ __launch_bounds__(kNT) void eom_accum_kernel(OctTables T, int Nw, int Nnu, int n_inu, int n_iw, int shift,
                                                         int fft_first, int numax, int wchunk, int kind, double U,
@@ -624,7 +627,7 @@ __global__ __launch_bounds__(kNT) void eom_accum_kernel(OctTables T, int Nw, int
                                                         const double2 *__restrict__ gd,
                                                         const double2 *__restrict__ l0,
                                                         const double2 *__restrict__ Grfft,
-                                                        double2 *__restrict__ partial) {
+                                                        double2 *__restrict__ partial, double *__restrict__ tpart) {
   extern __shared__ double2 smem[];
   double2 *buf = smem;
   double *cs = reinterpret_cast<double *>(smem + T.NO);
@@ -674,6 +677,27 @@ __global__ __launch_bounds__(kNT) void eom_accum_kernel(OctTables T, int Nw, int
   };
   int wi = next_w(w0);
   if (wi < w1) load_k(wi);
+  if (tpart) {  // this workgroup's share of sum_{q,w} kMult[q] chi_m^lam_m (loads in flight with the first K inputs)
+    const int total = T.Nq * Nw, step = gridDim.x * kNT;
+    double v = 0.0;
+    for (int i0 = L * kNT + threadIdx.x; i0 < total; i0 += 2 * step) {
+      const int i1 = i0 + step;
+      const double x0 = chim[i0], m0 = T.kmult[i0 % T.Nq];
+      const double x1 = i1 < total ? chim[i1] : 0.0, m1 = i1 < total ? T.kmult[i1 % T.Nq] : 0.0;
+      v = fma(m0, (lam_m != 0.0) ? x0 / (lam_m * x0 + 1.0) : x0, v);
+      v = fma(m1, (lam_m != 0.0) ? x1 / (lam_m * x1 + 1.0) : x1, v);
+    }
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    double *red = reinterpret_cast<double *>(buf);  // buf is first written after the loop's first barrier
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < kNT / 64; ++w) t += red[w];
+      tpart[L] = t;
+    }
+    __syncthreads();
+  }
   while (wi < w1) {
     const int wn = wi - n_iw;
     // G(nu + w) at this thread's points, loaded first: the passes hide the latency
@@ -709,18 +733,30 @@ __global__ __launch_bounds__(kNT) void eom_accum_kernel(OctTables T, int Nw, int
 }
 
 // Sums the chunk partials of one nu on the wedge, stores them on the orbits
-// and transforms back; adds the 1/(i nu) tail and the Hartree term.
+// and transforms back; adds the 1/(i nu) tail and the Hartree term.  The tail
+// (with ntp > 0) comes from the ntp partial k-omega sums of chi_m^lam_m, added
+// in order: sum_kw = (sum / Nk - tail_w + konst) / beta (DataTypes.jl:215-249),
+// tail_w the omega sum of the 1/(i w)^2 tail and konst = -beta^2 c2 / 12.
 template <int NQS, class PS>
 __global__ __launch_bounds__(kNT) void eom_final_kernel(OctTables T, int numax, int nchunks,
                                                         const double2 *__restrict__ partial, double scale, double U,
-                                                        double beta, double hartree, int add_tail,
-                                                        const double *__restrict__ tail_sum, double chiloc_m_sum,
+                                                        double beta, double hartree, const double *__restrict__ tpart,
+                                                        int ntp, double tail_w, double konst, double chiloc_m_sum,
                                                         double2 *__restrict__ sigma) {
   extern __shared__ double2 smem[];
+  __shared__ double tred[kNT / 64];
   double2 *buf = smem;
   double *cs = reinterpret_cast<double *>(smem + T.NO);
   const int nuii = blockIdx.x;
   for (int m = threadIdx.x; m < T.Ns; m += kNT) cs[m] = T.cosT[m];
+  // the tail's partial sums, reduced in a fixed order (thread strides, then
+  // the wave tree, then the waves in order after the passes' barriers)
+  if (ntp > 0) {
+    double t = 0.0;
+    for (int b = threadIdx.x; b < ntp; b += kNT) t += tpart[b];
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+    if ((threadIdx.x & 63) == 0) tred[threadIdx.x >> 6] = t;
+  }
   PS ps;
   ps.init(T);
   WedgeSlots<NQS> W;
@@ -756,7 +792,11 @@ __global__ __launch_bounds__(kNT) void eom_final_kernel(OctTables T, int numax, 
   ps.transform(buf, cs, T);
   // tail/(i nu_n) = -i * tail * beta / ((2n+1) pi)   (singleCore.jl:78-79)
   double tail = 0.0;
-  if (add_tail) tail = -U * (tail_sum[0] - chiloc_m_sum);
+  if (ntp > 0) {
+    double t = 0.0;
+    for (int w = 0; w < kNT / 64; ++w) t += tred[w];
+    tail = -U * ((t / T.Nk - tail_w + konst) / beta - chiloc_m_sum);
+  }
   const double tail_im = -tail * beta / ((2.0 * nuii + 1.0) * M_PI);
   double2 *out = sigma + (size_t)T.Nq * nuii;
 #pragma unroll
@@ -1297,28 +1337,35 @@ int launch_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int n
     return ldga_fail(ctx, LDGA_ERR_SHAPE, "nu window does not contain nu_n = 0 at every w");
   int wchunk = 1;
   const int nchunks = eom_chunks(ctx, Nw, numax, &wchunk);
+  const int nwg = numax * nchunks;
   double2 *partial = (double2 *)ldga_scratch(ctx, "eom_partial", (size_t)T.Nq * numax * nchunks * 16, st);
-  double *tsum = (double *)ldga_scratch(ctx, "eom_tail", 3 * 8, st);
-  if (!partial || !tsum) return LDGA_ERR_HIP;
   const bool add_tail = tc && eom_kind == LDGA_EOM_FULL;
-  if (add_tail) {
-    int rc = launch_sum_kw(ctx, chi_m, Nw, 0, Nw - 1, mp->chi_m_c2, mp->beta, lam_m, tsum, st);
-    if (rc) return rc;
+  // the accumulate workgroups' partial k-omega sums of chi_m for the tail (sum_kw over the whole omega range)
+  double *tpart = add_tail ? (double *)ldga_scratch(ctx, "eom_tail_part", (size_t)nwg * 8, st) : nullptr;
+  if (!partial || (add_tail && !tpart)) return LDGA_ERR_HIP;
+  // Re(c2/(i w_n)^2) = -c2 beta^2 / (4 pi^2 m^2), zero at m = 0, summed over the omega grid  (DataTypes.jl:223-225)
+  const double c2 = mp->chi_m_c2, beta = mp->beta;
+  double tail_w = 0.0;
+  for (int w = 0; w < Nw; ++w) {
+    const int m = w - fr->n_iw;
+    if (m != 0) tail_w += -c2 * beta * beta / (4.0 * M_PI * M_PI * (double)m * (double)m);
   }
+  const double konst = -beta * beta / 12.0 * c2;
   const size_t smem = (size_t)T.NO * 16 + (size_t)T.Ns * 8;
   const double2 *Grw = wedge_tables(ctx, fr, Grfft, "eom_grw", st);
   if (!Grw) return LDGA_ERR_HIP;
   int rc = with_cfg(ctx, [&](auto c, auto ps) {
-    return oct_launch(ctx, eom_accum_kernel<decltype(c)::value, decltype(ps)>, numax * nchunks, smem, st, T, Nw, Nnu, fr->n_inu,
+    return oct_launch(ctx, eom_accum_kernel<decltype(c)::value, decltype(ps)>, nwg, smem, st, T, Nw, Nnu, fr->n_inu,
                       fr->n_iw, fr->shift, fr->fft_first, numax, wchunk, eom_kind, mp->U, lam_m, lam_d, chi_m, chi_d,
-                      gamma_m, gamma_d, lambda0, Grw, partial);
+                      gamma_m, gamma_d, lambda0, Grw, partial, tpart);
   });
   if (rc) return rc;
   const double hartree = (eom_kind == LDGA_EOM_FULL || eom_kind == LDGA_EOM_REST) ? mp->n * mp->U / 2.0 : 0.0;
   const double scale = 1.0 / (mp->beta * T.Nk * T.Nk);
   return with_cfg(ctx, [&](auto c, auto ps) {
     return oct_launch(ctx, eom_final_kernel<decltype(c)::value, decltype(ps)>, numax, smem, st, T, numax, nchunks, partial, scale,
-                      mp->U, mp->beta, hartree, add_tail ? 1 : 0, tsum, chiloc_m_sum, sigma);
+                      mp->U, mp->beta, hartree, (const double *)tpart, add_tail ? nwg : 0, tail_w, konst,
+                      chiloc_m_sum, sigma);
   });
 }
 
