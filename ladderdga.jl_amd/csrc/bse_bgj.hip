@@ -169,6 +169,12 @@ __device__ __forceinline__ void bgj_solve(BgjShared<NT> &sh, const int lb, const
   // ---- M = A^T into the accumulators: M[i][c] = Gamma[c, i, w] + delta_ic / chi0[i]
   // (identity on the padding).  Row i = 16 wave + 4g + j, column c = 16t + n.
   // Rows and columns past N - 1 are clamped for the load and zeroed after it.
+  // Block 0's rows are needed in registers only for its diagonal tile (the
+  // owner inverts it before step 0, and step 0 replaces all of the owner's
+  // rows); the other tiles of those rows go to sh.rows, the operand of step
+  // 0, each loaded by the wave of its tile column.  So with NT > 1 wave 0
+  // loads one tile and starts inverting while the other waves load.
+  const bool lead = NT > 1 && wave == 0;
   AccPlane<float, NT> Vr, Vi;
   {
     constexpr int LG = 2;
@@ -192,7 +198,7 @@ __device__ __forceinline__ void bgj_solve(BgjShared<NT> &sh, const int lb, const
       double2 v[LG][4];
 #pragma unroll
       for (int tt = 0; tt < LG; ++tt) {
-        if (t0 + tt < NT) {
+        if (t0 + tt < NT && !(lead && t0 + tt > 0)) {
           const unsigned cofs = 16u * (unsigned)min(16 * (t0 + tt) + nl, N - 1);
 #pragma unroll
           for (int j = 0; j < 4; ++j) v[tt][j] = *reinterpret_cast<const double2 *>(Gb + (cofs + rofs[j]));
@@ -202,7 +208,7 @@ __device__ __forceinline__ void bgj_solve(BgjShared<NT> &sh, const int lb, const
       for (int tt = 0; tt < LG; ++tt) {
         const int t = t0 + tt;
         if (t < NT) {
-          const bool cok = 16 * t + n < N;
+          const bool cok = 16 * t + n < N && !(lead && t > 0);
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             const bool ok = cok && rb + j < N;
@@ -218,6 +224,19 @@ __device__ __forceinline__ void bgj_solve(BgjShared<NT> &sh, const int lb, const
         }
       }
       __asm__ volatile("" ::: "memory");
+    }
+    if (NT > 1 && wave > 0) {
+      // tile column `wave` of block 0's rows (no diagonal there) into sh.rows
+      const int c = 16 * wave + n;
+      const unsigned cofs = 16u * (unsigned)min(c, N - 1);
+      double2 b[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const double2 *>(Gb + (cofs + 16u * (unsigned)N * (unsigned)min(4 * g + j, N - 1)));
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const bool ok = c < N && 4 * g + j < N;
+        sh.rows[4 * g + j][c] = make_float2(ok ? (float)b[j].x : 0.0f, ok ? (float)b[j].y : 0.0f);
+      }
     }
   }
 
@@ -401,8 +420,8 @@ __device__ __forceinline__ void bgj_solve(BgjShared<NT> &sh, const int lb, const
           const float2 a = sh.rows[Ln][16 * wave + 4 * c + Lg];
           bf_cmma(bf_prep_a(a.x, a.y), sh.Wb[c][k + 1][Ll], dtr, dti);
         }
-      } else {
-        publish_rows();
+      } else if (NT == 1) {
+        publish_rows();  // (with NT > 1 the other waves loaded block 0's rows into sh.rows)
       }
       invert_diag(dtr, dti);
       __builtin_amdgcn_s_setprio(0);
