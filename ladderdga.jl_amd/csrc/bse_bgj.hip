@@ -312,7 +312,8 @@ __device__ __forceinline__ void bgj_solve(BgjShared<NT> &sh, const int lb, const
 #pragma unroll
     for (int j = 0; j < 4; ++j) sh.P[4 * Lg + j][Ln] = make_float2(dr[j], di[j]);
   };
-  // k = -1: only the owner of block 0 publishes its rows and inverts its block
+  // k = -1: only the owner of block 0 inverts its block (and, with NT = 1,
+  // publishes its rows)
   for (int k = -1; k < NB; ++k) {
     __asm__ volatile("" : "+v"(Lg), "+v"(Ln), "+v"(Ll));
     const bool next_owner = wave == k + 1 && k + 1 < NB;
