@@ -270,7 +270,7 @@ __device__ __forceinline__ void bgj_solve(BgjShared<NT> &sh, const int lb, const
   // past N - 1 are identity rows with zeros in the matrix columns: their
   // steps have piv = 1 and change nothing (exact products by 0 and 1).
   auto invert_diag = [&](const v4 tr, const v4 ti) {
-    float dr[4], di[4];
+    float dr[4], di[4], dmin = 1.0f;  // dmin: the smallest |piv|^2 (a zero pivot is flagged once, after the steps)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       dr[j] = tr[j];
@@ -290,7 +290,7 @@ __device__ __forceinline__ void bgj_solve(BgjShared<NT> &sh, const int lb, const
       }
       // 1 / piv = conj(piv) / |piv|^2, reciprocal refined by a Newton step
       const float dd = px * px + py * py;
-      zero_seen |= dd == 0.0f;
+      dmin = fminf(dmin, dd);
       float rd = fast_rcp(dd);
       rd = fmaf(rd, fmaf(-dd, rd, 1.0f), rd);
       const float ivx = px * rd, ivy = -py * rd;
@@ -309,6 +309,7 @@ __device__ __forceinline__ void bgj_solve(BgjShared<NT> &sh, const int lb, const
         di[j] = y;
       }
     });
+    zero_seen |= dmin == 0.0f;
 #pragma unroll
     for (int j = 0; j < 4; ++j) sh.P[4 * Lg + j][Ln] = make_float2(dr[j], di[j]);
   };
