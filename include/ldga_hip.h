@@ -38,7 +38,8 @@ extern "C" {
 #define LDGA_PREC_FP64 0  /* FP64 Gauss-Jordan elimination                     */
 #define LDGA_PREC_MIXED 1 /* FP32 factorisation + FP64 iterative refinement    */
 
-/* EoM kernel selectors: src/ladderDGATools.jl:260-266 (eom, eom_chi_m, ...) */
+/* EoM kernel selectors: src/ladderDGATools.jl:260-271 (eom, eom_χ_m, ..., eom_sp_01, eom_sp_02, eom_sp, eom_ch,
+ * eom_rest; the real-χ methods) */
 #define LDGA_EOM_FULL 0
 #define LDGA_EOM_CHI_M 1
 #define LDGA_EOM_GAMMA_M 2
@@ -46,6 +47,10 @@ extern "C" {
 #define LDGA_EOM_GAMMA_D 4
 #define LDGA_EOM_REST_01 5
 #define LDGA_EOM_REST 6
+#define LDGA_EOM_SP_01 7
+#define LDGA_EOM_SP_02 8
+#define LDGA_EOM_SP 9
+#define LDGA_EOM_CH 10
 
 typedef struct ldga_ctx ldga_ctx;
 
@@ -218,9 +223,10 @@ int ldga_sum_w_dev(ldga_ctx *ctx, const void *chi, int Nq, int Nw, int w_lo, int
  *   (1/beta) sum_w conv_q(K(q,nu,w), G(k+q, nu+w)) + tail/(i nu_n) + n U/2,
  * K = eom(U, gamma_m, gamma_d, chi_m^λm, chi_d^λd, lambda0)
  * (src/ladderDGATools.jl:260), tail = -U (sum_kω(chi_m^λm) - chiloc_m_sum)
- * (tail coefficient mp->chi_m_c2) when tc != 0.  eom_kind selects one of the split kernels (LDGA_EOM_*) used
- * by calc_Σ_parts; for those the Hartree term is added only to REST and the
- * tail only when tc != 0 and eom_kind == FULL.  chi_m/chi_d (Nq x Nw) are
+ * (tail coefficient mp->chi_m_c2) when tc != 0.  eom_kind selects another of the reference's EoM functions
+ * (LDGA_EOM_*: the parts of calc_Σ_parts, and the eomf argument of calc_Σ_ω!,
+ * src/ladderDGATools_singleCore.jl:49-69); the Hartree term is added to FULL
+ * and REST only, the tail only when tc != 0 and eom_kind == FULL.  chi_m/chi_d (Nq x Nw) are
  * read-only: the lambda map is applied on the fly and never stored. */
 int ldga_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int numax, double lam_m, double lam_d,
                int tc, int eom_kind, double chiloc_m_sum, const double *chi_m, const double *gamma_m,

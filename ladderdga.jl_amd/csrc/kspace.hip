@@ -602,6 +602,15 @@ __device__ __forceinline__ double2 eom_value(int kind, double U, double2 gm, dou
       return cz(-U, 0.0);
     case LDGA_EOM_REST:
       return cscale(l0, U);
+    // src/ladderDGATools.jl:267-270
+    case LDGA_EOM_SP_01:
+      return cscale(cz(gm.x * 0.5 * (1.0 + U * cm) - 0.5, gm.y * 0.5 * (1.0 + U * cm)), U);
+    case LDGA_EOM_SP_02:
+      return cscale(cz(gm.x * (1.0 + U * cm) - 1.0, gm.y * (1.0 + U * cm)), U);
+    case LDGA_EOM_SP:
+      return cscale(cz(gm.x * 1.5 * (1.0 + U * cm) - 1.5, gm.y * 1.5 * (1.0 + U * cm)), U);
+    case LDGA_EOM_CH:
+      return cscale(cz(gd.x * 0.5 * (1.0 - U * cd) - 0.5, gd.y * 0.5 * (1.0 - U * cd)), -U);
     default: {
       const double am = 1.5 * (1.0 + U * cm), ad = 0.5 * (1.0 - U * cd);
       return cscale(cz(gm.x * am - gd.x * ad - 1.5 + 0.5 + l0.x, gm.y * am - gd.y * ad + l0.y), U);
@@ -1328,7 +1337,7 @@ int launch_sigma(ldga_ctx *ctx, const ldga_freq *fr, const ldga_model *mp, int n
   const OctTables &T = ctx->oct;
   const int Nnu = 2 * fr->n_inu, Nw = 2 * fr->n_iw + 1;
   if (numax < 1 || numax > 2 * fr->n_inu) return ldga_fail(ctx, LDGA_ERR_SHAPE, "numax=%d out of range", numax);
-  if (eom_kind < LDGA_EOM_FULL || eom_kind > LDGA_EOM_REST) return ldga_fail(ctx, LDGA_ERR_ARG, "bad eom kind");
+  if (eom_kind < LDGA_EOM_FULL || eom_kind > LDGA_EOM_CH) return ldga_fail(ctx, LDGA_ERR_ARG, "bad eom kind");
   // G(nu + w) must lie inside fft_range for every nu in 0..numax-1
   if (-fr->n_iw < fr->fft_first || numax - 1 + fr->n_iw > fr->fft_first + fr->n_fft - 1)
     return ldga_fail(ctx, LDGA_ERR_SHAPE, "fft_range does not cover nu+w");

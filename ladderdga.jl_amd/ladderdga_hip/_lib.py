@@ -23,7 +23,8 @@ LDGA_ERR_STATE = -5
 PREC_FP64 = 0
 PREC_MIXED = 1
 
-EOM_KINDS = {"full": 0, "χ_m": 1, "γ_m": 2, "χ_d": 3, "γ_d": 4, "rest_01": 5, "rest": 6}
+EOM_KINDS = {"full": 0, "χ_m": 1, "γ_m": 2, "χ_d": 3, "γ_d": 4, "rest_01": 5, "rest": 6,
+             "sp_01": 7, "sp_02": 8, "sp": 9, "ch": 10}
 
 # every symbol include/ldga_hip.h declares
 EXPORTED = [

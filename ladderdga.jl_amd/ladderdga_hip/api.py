@@ -342,6 +342,23 @@ def calc_Σ(χm: χT, γm: γT, χd: χT, γd: γT, χ_m_sum: float, λ0, Gνω_
     return _sigma_call(ctx, sP, mP, νmax, λm, λd, tc, 0, χ_m_sum, χm, γm, χd, γd, λ0, Gνω_r, kG.Nq)
 
 
+def calc_Σ_ω(eomf: str, χm: χT, γm: γT, χd: χT, γd: γT, Gνω_r, λ0, U: float, kG: KGrid,
+             sP: SimulationParameters, νmax: int | None = None, ctx=None):
+    """Σ_ladder[q, ν] = Σ_ω conv_q(K, G_rfft(ν + ω)), K = eomf(U, γm, γd, χm, χd, λ₀), without 1/β, tail or
+    Hartree term (calc_Σ_ω!, src/ladderDGATools_singleCore.jl:49-69).  eomf names one of the reference's
+    real-χ EoM functions (src/ladderDGATools.jl:260-271): "full" (eom), "χ_m", "γ_m", "χ_d", "γ_d",
+    "rest_01", "rest", "sp_01", "sp_02", "sp", "ch" (eom_χ_m, ..., eom_ch).  χ is used as given (apply λ to
+    it beforehand, as the reference's caller does)."""
+    if eomf not in _lib.EOM_KINDS:
+        raise ValueError(f"unknown EoM function {eomf!r}; one of {sorted(_lib.EOM_KINDS)}")
+    ctx = _ctx(ctx, kG)
+    νmax = sP.n_iν if νmax is None else νmax
+    # the kernel's scale is 1/β and it adds n U/2 only for "full" and "rest": β = 1 and n = 0 give the bare sum
+    bare = ModelParameters(U=U, μ=0.0, β=1.0, n=0.0)
+    return _sigma_call(ctx, sP, bare, νmax, 0.0, 0.0, False, _lib.EOM_KINDS[eomf], 0.0, χm, γm, χd, γd, λ0, Gνω_r,
+                       kG.Nq)
+
+
 def calc_Σ_parts(χm: χT, γm: γT, χd: χT, γd: γT, χ_m_sum: float, λ0, Gνω_r, kG: KGrid, mP: ModelParameters,
                  sP: SimulationParameters, tc: bool = True, λm: float = 0.0, λd: float = 0.0, ctx=None):
     """The 7 contributions χ_m, γ_m, χ_d, γ_d, U, rest(+Hartree), tail (src/ladderDGATools_singleCore.jl:153-189)."""

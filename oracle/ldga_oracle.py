@@ -516,6 +516,11 @@ EOM = {
     "γ_d": lambda U, gm, gd, cm, cd, l0: -U * (gd * 0.5) + 0 * gd,
     "rest_01": lambda U, gm, gd, cm, cd, l0: (-U * 1.0 + 0.0j) + 0 * gm,
     "rest": lambda U, gm, gd, cm, cd, l0: U * l0,
+    # src/ladderDGATools.jl:267-270
+    "sp_01": lambda U, gm, gd, cm, cd, l0: U * (gm * 0.5 * (1 + U * cm) - 0.5),
+    "sp_02": lambda U, gm, gd, cm, cd, l0: U * (gm * 1.0 * (1 + U * cm) - 1.0),
+    "sp": lambda U, gm, gd, cm, cd, l0: U * (gm * 1.5 * (1 + U * cm) - 1.5),
+    "ch": lambda U, gm, gd, cm, cd, l0: -U * (gd * 0.5 * (1 - U * cd) - 0.5),
 }
 
 

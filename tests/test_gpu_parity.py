@@ -367,6 +367,25 @@ def test_sigma_parts(ctx, name, kw):
     assert relerr(P.sum(axis=2), full) <= TOL
 
 
+@pytest.mark.parametrize("name,kw", [case(0), case(2)])
+def test_sigma_omega_every_eom(ctx, name, kw):
+    """calc_Σ_ω! (ladderDGATools_singleCore.jl:49-69) with each of the reference's real-χ EoM functions
+    (ladderDGATools.jl:260-271, eom_sp_01/_02, eom_sp and eom_ch included): the bare ω sum against the
+    oracle's loop nest."""
+    import ladderdga_hip as L
+    inp, oK, omP, osP, _, cm, cd, gm, gd, l0 = _ladder_inputs(name, kw)
+    νmax = inp.sP.n_iν
+    for kind in L._lib.EOM_KINDS:
+        print(f"calc_Σ_ω {name} {kind}", flush=True)
+        got = L.calc_Σ_ω(kind, L.χT(cm, inp.mP.β), L.γT(gm), L.χT(cd, inp.mP.β), L.γT(gd), inp.G_rfft, l0,
+                         inp.mP.U, inp.kG, inp.sP, ctx=ctx)
+        ref = O.calc_Σ_ω(O.EOM[kind], cm, gm, cd, gd, inp.G_rfft, l0, omP.U, oK, osP, νmax)
+        assert relerr(got, ref) <= TOL, kind
+    with pytest.raises(ValueError, match="unknown EoM"):
+        L.calc_Σ_ω("nope", L.χT(cm, inp.mP.β), L.γT(gm), L.χT(cd, inp.mP.β), L.γT(gd), inp.G_rfft, l0,
+                   inp.mP.U, inp.kG, inp.sP, ctx=ctx)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_pipeline_2d16(ctx, precision):
     """bubble -> BSE (m, d) -> λ₀ -> Σ on the 16x16 workload, against the oracle's chain."""

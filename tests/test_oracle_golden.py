@@ -317,6 +317,25 @@ def test_eom_brute_force_and_identities():
     assert np.allclose(parts.sum(axis=2), Σ_all, atol=1e-12)
 
 
+def test_eom_split_functions_identities():
+    """The split EoM functions of ladderDGATools.jl:260-270 against each other (random complex γ, λ₀ and
+    real χ): eom = eom_sp + eom_ch + eom_rest; eom_sp = eom_χ_m + eom_γ_m + 1.5 eom_rest_01; eom_ch =
+    eom_χ_d + eom_γ_d - 0.5 eom_rest_01; eom_sp_01 = eom_sp / 3 and eom_sp_02 = 2 eom_sp / 3.  The full
+    eom and the χ/γ/rest parts are pinned by the brute-force EoM and the parts sum above, so a wrong
+    coefficient or sign in eom_sp_01, eom_sp_02, eom_sp or eom_ch fails here."""
+    rng = np.random.default_rng(7)
+    n = 64
+    U = 1.7
+    gm, gd, l0 = (rng.standard_normal(n) + 1j * rng.standard_normal(n) for _ in range(3))
+    cm, cd = rng.uniform(-0.5, 0.8, n), rng.uniform(-0.5, 0.8, n)
+    E = {k: f(U, gm, gd, cm, cd, l0) for k, f in O.EOM.items()}
+    assert np.allclose(E["full"], E["sp"] + E["ch"] + E["rest"], atol=1e-13)
+    assert np.allclose(E["sp"], E["χ_m"] + E["γ_m"] + 1.5 * E["rest_01"], atol=1e-13)
+    assert np.allclose(E["ch"], E["χ_d"] + E["γ_d"] - 0.5 * E["rest_01"], atol=1e-13)
+    assert np.allclose(E["sp_01"], E["sp"] / 3, atol=1e-13)
+    assert np.allclose(E["sp_02"], 2 * E["sp"] / 3, atol=1e-13)
+
+
 def test_calc_E_limits():
     """U -> 0: E_kin -> 2/Nk sum_k eps_k f(eps_k - mu); E_pot with a constant Sigma = Sigma_h * n_sigma."""
     kG = O.KGrid(2, 8, 1.0)
