@@ -37,7 +37,7 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32
 # HBM bytes per BSE launch, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script, each
 # scaled by what the counter reads for a 1024 MiB device copy (--copy-calib): fetch x2.000, write x1.000.
 # Summary and commands: profiles/bse_mixed_fp64_2d64.txt.
-MEASURED_TRAFFIC = {("2d64", "mixed"): 303.6e6, ("2d64", "fp64"): 292.2e6}
+MEASURED_TRAFFIC = {("2d64", "mixed"): 301.5e6, ("2d64", "fp64"): 292.2e6}
 
 
 def parse():
