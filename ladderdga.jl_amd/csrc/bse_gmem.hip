@@ -20,9 +20,14 @@
 
 namespace {
 
-constexpr int GM_THREADS = 256;
+// NTH threads per matrix: 256 when the launch covers every (q, w) (several
+// workgroups share a CU), 1024 for a short re-solve list (a few workgroups on
+// an otherwise idle device).  The step's update walks the matrix by flat index
+// with GM_UNROLL loads in flight per thread.
+constexpr int GM_UNROLL = 4;
 
-__global__ __launch_bounds__(GM_THREADS) void bse_gmem_f64_kernel(int N, int Nq, int Nc, int shell,
+template <int NTH>
+__global__ __launch_bounds__(NTH) void bse_gmem_f64_kernel(int N, int Nq, int Nc, int shell,
                                                                   const double2 *__restrict__ Gam,
                                                                   const double2 *__restrict__ chi0, double s,
                                                                   double U, double beta, double *__restrict__ chi_out,
@@ -38,15 +43,16 @@ __global__ __launch_bounds__(GM_THREADS) void bse_gmem_f64_kernel(int N, int Nq,
   double2 *y = fcol + N;     // solution
   int *pivoted = reinterpret_cast<int *>(y + N);
   int *pcol = pivoted + N;   // pivot column of each row
-  __shared__ double red_v[GM_THREADS / 64];
-  __shared__ int red_c[GM_THREADS / 64];
+  __shared__ double red_v[NTH / 64];
+  __shared__ int red_c[NTH / 64];
   __shared__ int piv_col;
-  __shared__ double red_chi[GM_THREADS / 64];
+  __shared__ double red_chi[NTH / 64];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int NN = N * N;
   double2 *M = ws + (size_t)blockIdx.x * N * N;
   // this thread's first (row, column) of the flat row-major walk, and the step
-  const int i0 = tid / N, c0 = tid - i0 * N, di = GM_THREADS / N, dc = GM_THREADS - di * N;
+  const int i0 = tid / N, c0 = tid - i0 * N, di = NTH / N, dc = NTH - di * N;
   // a list whose length is known only on the device: at most n_items of it
   if (n_dev) n_items = min(n_items, __builtin_amdgcn_readfirstlane((int)*n_dev));
 
@@ -66,7 +72,7 @@ __global__ __launch_bounds__(GM_THREADS) void bse_gmem_f64_kernel(int N, int Nq,
         ++i;
       }
     }
-    for (int c = tid; c < N; c += GM_THREADS) pivoted[c] = 0;
+    for (int c = tid; c < N; c += NTH) pivoted[c] = 0;
     int zero = 0;
     __syncthreads();
 
@@ -76,7 +82,7 @@ __global__ __launch_bounds__(GM_THREADS) void bse_gmem_f64_kernel(int N, int Nq,
       // a NaN counts as the largest value (its column is taken, the NaN propagates)
       double bv = -1.0;
       int bc = N;
-      for (int c = tid; c < N; c += GM_THREADS) {
+      for (int c = tid; c < N; c += NTH) {
         if (pivoted[c]) continue;
         const double2 m = Mr[c];
         double v = fabs(m.x) + fabs(m.y);
@@ -102,7 +108,7 @@ __global__ __launch_bounds__(GM_THREADS) void bse_gmem_f64_kernel(int N, int Nq,
       if (tid == 0) {
         double v = red_v[0];
         int c = red_c[0];
-        for (int k = 1; k < GM_THREADS / 64; ++k)
+        for (int k = 1; k < NTH / 64; ++k)
           if (red_v[k] > v || (red_v[k] == v && red_c[k] < c)) {
             v = red_v[k];
             c = red_c[k];
@@ -116,22 +122,30 @@ __global__ __launch_bounds__(GM_THREADS) void bse_gmem_f64_kernel(int N, int Nq,
       const double2 piv = Mr[p];
       zero |= piv.x == 0.0 && piv.y == 0.0;
       const double2 ipiv = crecip(piv);
-      for (int c = tid; c < N; c += GM_THREADS) rs[c] = cmul(Mr[c], ipiv);
-      for (int i = tid; i < N; i += GM_THREADS) fcol[i] = M[(size_t)N * i + p];
+      for (int c = tid; c < N; c += NTH) rs[c] = cmul(Mr[c], ipiv);
+      for (int i = tid; i < N; i += NTH) fcol[i] = M[(size_t)N * i + p];
       __syncthreads();
-      for (int i = i0, c = c0; i < N;) {
-        double2 &m = M[(size_t)N * i + c];
-        if (i == r)
-          m = c == p ? ipiv : cz(-rs[c].x, -rs[c].y);
-        else if (c == p)
-          m = cmul(fcol[i], ipiv);
-        else
-          m = csub(m, cmul(fcol[i], rs[c]));
-        c += dc;
-        i += di;
-        if (c >= N) {
-          c -= N;
-          ++i;
+      for (int k0 = tid; k0 < NN; k0 += GM_UNROLL * NTH) {
+        double2 mv[GM_UNROLL];
+#pragma unroll
+        for (int u = 0; u < GM_UNROLL; ++u) {
+          const int k = k0 + u * NTH;
+          if (k < NN) mv[u] = M[k];
+        }
+#pragma unroll
+        for (int u = 0; u < GM_UNROLL; ++u) {
+          const int k = k0 + u * NTH;
+          if (k < NN) {
+            const int i = k / N, c = k - i * N;
+            double2 m;
+            if (i == r)
+              m = c == p ? ipiv : cz(-rs[c].x, -rs[c].y);
+            else if (c == p)
+              m = cmul(fcol[i], ipiv);
+            else
+              m = csub(mv[u], cmul(fcol[i], rs[c]));
+            M[k] = m;
+          }
         }
       }
       __syncthreads();
@@ -141,28 +155,28 @@ __global__ __launch_bounds__(GM_THREADS) void bse_gmem_f64_kernel(int N, int Nq,
     // column-major per (q, w) as calc_gen_χ returns it (BSETools.jl:259-275)
     if (inv_out) {
       double2 *X = inv_out + (size_t)(q + (size_t)Nq * w) * N * N;
-      for (int k = tid; k < N * N; k += GM_THREADS) {
+      for (int k = tid; k < N * N; k += NTH) {
         const int i = k % N, r = k / N;
         X[i + (size_t)N * pcol[r]] = M[(size_t)N * r + pcol[i]];
       }
     }
     // y[p_r] = sum_c M[r][c]; chi = Re sum y / beta^2; gamma = y / (chi0 (1 + s U chi))
-    for (int r = tid; r < N; r += GM_THREADS) {
+    for (int r = tid; r < N; r += NTH) {
       double2 acc = cz(0.0, 0.0);
       for (int c = 0; c < N; ++c) acc = cadd(acc, M[(size_t)N * r + c]);
       y[pcol[r]] = acc;
     }
     __syncthreads();
     double part = 0.0;
-    for (int k = tid; k < N; k += GM_THREADS) part += y[k].x;
+    for (int k = tid; k < N; k += NTH) part += y[k].x;
     for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
     if (lane == 0) red_chi[wave] = part;
     __syncthreads();
     double tot = 0.0;
-    for (int k = 0; k < GM_THREADS / 64; ++k) tot += red_chi[k];
+    for (int k = 0; k < NTH / 64; ++k) tot += red_chi[k];
     const double chi = tot / (beta * beta);
     const double fac = 1.0 + s * U * chi;
-    for (int k = tid; gamma_out && k < N; k += GM_THREADS) {
+    for (int k = tid; gamma_out && k < N; k += NTH) {
       const double2 c0v = chi0[q + (size_t)Nq * (k + (size_t)Nc * w)];  // no shell offset (BSETools.jl:236)
       gamma_out[q + (size_t)Nq * (k + (size_t)N * w)] = cdiv(y[k], cscale(c0v, fac));
     }
@@ -194,9 +208,14 @@ int launch_bse_gmem_f64(ldga_ctx *ctx, int N, int Nq, int Nw, int Nc, int shell,
   double2 *ws = (double2 *)ldga_scratch(ctx, "bse_gmem_ws", (size_t)grid * slot, st);
   if (!ws) return LDGA_ERR_HIP;
   const size_t lds = (size_t)N * (3 * sizeof(double2) + 2 * sizeof(int));
-  bse_gmem_f64_kernel<<<dim3((unsigned)grid), dim3(GM_THREADS), lds, st>>>(N, Nq, Nc, shell, Gam, chi0, s, U, beta,
-                                                                          chi, gamma, cnt, only, n_items, n_dev, ws,
-                                                                          inv);
+  if (only)  // a re-solve list: few matrices, each on a whole CU
+    bse_gmem_f64_kernel<1024><<<dim3((unsigned)grid), dim3(1024), lds, st>>>(N, Nq, Nc, shell, Gam, chi0, s, U,
+                                                                             beta, chi, gamma, cnt, only, n_items,
+                                                                             n_dev, ws, inv);
+  else
+    bse_gmem_f64_kernel<256><<<dim3((unsigned)grid), dim3(256), lds, st>>>(N, Nq, Nc, shell, Gam, chi0, s, U,
+                                                                           beta, chi, gamma, cnt, only, n_items,
+                                                                           n_dev, ws, inv);
   LDGA_HIP_TRY(ctx, hipGetLastError());
   return LDGA_OK;
 }
