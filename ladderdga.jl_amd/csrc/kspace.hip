@@ -514,10 +514,14 @@ __device__ __forceinline__ void orbit_store(double2 *buf, const OctTables &T, in
 // a cache line per point (the wedge is 1/2^D/D! of the grid), on the wedge it
 // is one contiguous Nq-vector that stays in L2.
 // ---------------------------------------------------------------------------
-__global__ void wedge_tables_kernel(OctTables T, int nplanes, const double2 *__restrict__ src,
-                                    double2 *__restrict__ dst) {
+// (blockIdx.z selects the table: one launch compacts up to two tables)
+__global__ void wedge_tables_kernel(OctTables T, int nplanes, const double2 *__restrict__ src0,
+                                    double2 *__restrict__ dst0, const double2 *__restrict__ src1,
+                                    double2 *__restrict__ dst1) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x, f = blockIdx.y;
   if (q >= T.Nq || f >= nplanes) return;
+  const double2 *src = blockIdx.z ? src1 : src0;
+  double2 *dst = blockIdx.z ? dst1 : dst0;
   dst[q + (size_t)T.Nq * f] = src[T.wedgefull[q] + (size_t)T.Nk * f];
 }
 
@@ -1171,17 +1175,24 @@ static int with_cfg(ldga_ctx *ctx, F &&f) {
 }
 
 // the full-grid G table src (Nk x n_fft) on the wedge, in the context's scratch of the given role
-static const double2 *wedge_tables(ldga_ctx *ctx, const ldga_freq *fr, const double2 *src, const char *role,
-                                   hipStream_t st) {
+// the full-grid tables src0 (and src1, when given) on the wedge, in one launch;
+// returns the compacted src0 (*dst1 receives the compacted src1)
+static const double2 *wedge_tables(ldga_ctx *ctx, const ldga_freq *fr, const double2 *src0, const char *role0,
+                                   hipStream_t st, const double2 *src1 = nullptr, const char *role1 = nullptr,
+                                   const double2 **dst1 = nullptr) {
   const OctTables &T = ctx->oct;
-  double2 *dst = (double2 *)ldga_scratch(ctx, role, (size_t)T.Nq * fr->n_fft * 16, st);
-  if (!dst) return nullptr;
-  wedge_tables_kernel<<<dim3((T.Nq + 255) / 256, fr->n_fft), 256, 0, st>>>(T, fr->n_fft, src, dst);
+  const size_t bytes = (size_t)T.Nq * fr->n_fft * 16;
+  double2 *d0 = (double2 *)ldga_scratch(ctx, role0, bytes, st), *d1 = nullptr;
+  if (!d0) return nullptr;
+  if (src1 && !(d1 = (double2 *)ldga_scratch(ctx, role1, bytes, st))) return nullptr;
+  wedge_tables_kernel<<<dim3((T.Nq + 255) / 256, fr->n_fft, src1 ? 2 : 1), 256, 0, st>>>(T, fr->n_fft, src0, d0,
+                                                                                         src1, d1);
   if (hipGetLastError() != hipSuccess) {
     ldga_fail(ctx, LDGA_ERR_HIP, "wedge_tables_kernel launch failed");
     return nullptr;
   }
-  return dst;
+  if (dst1) *dst1 = d1;
+  return d0;
 }
 
 int launch_bubble(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double2 *Gfft, const double2 *Grfft,
@@ -1201,7 +1212,8 @@ int launch_bubble(ldga_ctx *ctx, const ldga_freq *fr, double beta, const double2
                      fr->fft_first + fr->n_fft - 1, lo, hi);
   const size_t smem = (size_t)T.NO * 16 + (size_t)T.Ns * 8;
   const double scale = -beta / (T.Nk * T.Nk);
-  const double2 *Gw = wedge_tables(ctx, fr, Gfft, "bubble_gw", st), *Grw = wedge_tables(ctx, fr, Grfft, "bubble_grw", st);
+  const double2 *Grw = nullptr;
+  const double2 *Gw = wedge_tables(ctx, fr, Gfft, "bubble_gw", st, Grfft, "bubble_grw", &Grw);
   if (!Gw || !Grw) return LDGA_ERR_HIP;
   // one round of resident workgroups, each over a contiguous range of pairs
   return with_cfg(ctx, [&](auto c, auto ps) {
